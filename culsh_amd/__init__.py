@@ -1,0 +1,9 @@
+"""culsh_amd — MI355X-native LSH (p-stable, multiprobe) kNN index/query path.
+
+A drop-in for the cuLSH HashTables/SearchTables path: hand-written gfx950 HIP
+kernels in ``lib/libculsh.so`` behind the C ABI of ``include/culsh_c.h``,
+with this package as the Python mirror of the reference classes.
+"""
+from .lsh import FORM_PLAIN, FORM_PRECISION, HashTables, SearchTables  # noqa: F401
+
+__all__ = ["HashTables", "SearchTables", "FORM_PLAIN", "FORM_PRECISION"]

@@ -1,0 +1,886 @@
+// libculsh — MI355X-native LSH index/query path behind a C ABI.
+//
+// Host orchestration for cuLSH::HashTables (cuLSH_HashTables.cu:51-520) and
+// cuLSH::SearchTables (cuLSH_SearchTables.cu:42-463).  The index lives in
+// HBM: dataset, bucket keys, offsets, member ids, codes and a key directory;
+// both index() and search() run as a short chain of device kernels on one
+// stream per object.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/culsh_c.h"
+#include "common.hpp"
+#include "index_kernels.hpp"
+#include "philox.hpp"
+#include "search_kernels.hpp"
+#include "sort.hpp"
+
+using namespace culsh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(CULSH_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                     \
+  } while (0)
+
+#define TRY(expr)               \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != CULSH_OK) return rc_; \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Grow-only allocation of at least `count` elements.
+  int ensure(size_t count) {
+    if (count <= n && p) return CULSH_OK;
+    release();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CULSH_E_MEMORY, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    n = std::max<size_t>(count, 1);
+    return CULSH_OK;
+  }
+  int exact(size_t count) {
+    release();
+    return ensure(count);
+  }
+};
+
+int ceil_log2_host(long long x) {
+  int l = 0;
+  while ((1ll << l) < x) ++l;
+  return l;
+}
+
+int floor_log2_host(long long x) {
+  int l = 0;
+  while ((2ll << l) <= x) ++l;
+  return l;
+}
+
+inline unsigned grid_for(long long n, int block = 256) {
+  return (unsigned)((n + block - 1) / block);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// HashTables
+// ---------------------------------------------------------------------------
+struct culsh_ht {
+  int N = 0, D = 0, L = 0, M = 0;
+  float W = 0.f;
+  int form = CULSH_FORM_PLAIN;
+  bool has_params = false;
+  std::vector<float> A, b;  // [L][D*M] (column-major per table), [L][M]
+  hipStream_t stream = nullptr;
+
+  // Projection parameters on the device.
+  int J = 0, Jp = 0;
+  DevBuf<float> dAd;      // [D][Jp]   column j = t*M + m
+  DevBuf<float> dAorig;   // [J][D]    column j contiguous over d
+  DevBuf<float> dbj;      // [Jp]
+  DevBuf<float> dnaj;     // [Jp]      upper bound of ||a_j||_2
+  DevBuf<double> dAt64;   // [L][D][M]
+  DevBuf<double> db64;    // [L][M]
+  DevBuf<uint64_t> dseeds;
+  uint64_t hash_seed = 0x243F6A8885A308D3ull;
+
+  // Index.
+  bool indexed = false;
+  DevBuf<float> dX_own;
+  const float* dX = nullptr;          // resident dataset [N][D]
+  const float* hX_indexed = nullptr;  // host pointer index() was given
+  DevBuf<uint64_t> ukeys;             // [B_total] sorted unique keys
+  DevBuf<uint32_t> bstart;            // [B_total + 1] offsets into sids
+  DevBuf<uint32_t> sids;              // [L*N] members, bucket by bucket
+  DevBuf<int16_t> bcodes;             // [B_total][M]
+  DevBuf<uint32_t> dir;               // [nslots + 1]
+  std::vector<uint32_t> table_base;   // [L + 1] first global bucket of each table
+  int tb = 1, P = 1, shift = 0;
+  long long nslots = 0;
+  unsigned long long rechecked = 0;
+
+  // Lexicographic numbering and host mirrors, built on demand.
+  bool canon = false;
+  DevBuf<int> renumber;  // global bucket -> table-local lexicographic index
+  std::vector<unsigned> hB;
+  std::vector<std::vector<unsigned>> hTotal, hStart, hSize;
+  std::vector<std::vector<float>> hCodes;
+
+  ~culsh_ht() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  long long B_total() const { return table_base.empty() ? 0 : (long long)table_base[L]; }
+  void drop_index() {
+    indexed = false;
+    canon = false;
+    dX_own.release();
+    dX = nullptr;
+    hX_indexed = nullptr;
+    ukeys.release();
+    bstart.release();
+    sids.release();
+    bcodes.release();
+    dir.release();
+    renumber.release();
+    table_base.clear();
+    hB.clear();
+    hTotal.clear();
+    hStart.clear();
+    hSize.clear();
+    hCodes.clear();
+    rechecked = 0;
+  }
+};
+
+namespace {
+
+int ht_stream(culsh_ht* ht) {
+  if (!ht->stream) HIP_TRY(hipStreamCreateWithFlags(&ht->stream, hipStreamNonBlocking));
+  return CULSH_OK;
+}
+
+bool params_ok(int N, int D, int L, int M, float W) {
+  return N > 0 && D > 0 && L > 0 && M > 0 && W > 0.0f;
+}
+
+int check_limits(int N, int D, int L, int M) {
+  if (M > 64) return fail(CULSH_E_LIMIT, "M = %d exceeds the supported maximum of 64", M);
+  if ((long long)L * N >= (1ll << 31))
+    return fail(CULSH_E_LIMIT, "L*N = %lld exceeds 2^31 - 1", (long long)L * N);
+  if ((long long)L * M >= (1 << 24)) return fail(CULSH_E_LIMIT, "L*M exceeds 2^24");
+  if (D > (1 << 20)) return fail(CULSH_E_LIMIT, "D exceeds 2^20");
+  return CULSH_OK;
+}
+
+// Uploads the projection parameters in the layouts the kernels read.
+int upload_params(culsh_ht* ht) {
+  TRY(ht_stream(ht));
+  const int D = ht->D, L = ht->L, M = ht->M;
+  ht->J = L * M;
+  ht->Jp = (ht->J + kPjCols - 1) / kPjCols * kPjCols;
+  const int Jp = ht->Jp;
+  std::vector<float> Ad((size_t)D * Jp, 0.f), Aorig((size_t)ht->J * D), bj(Jp, 0.f), naj(Jp, 0.f);
+  std::vector<double> At64((size_t)L * D * M), b64((size_t)L * M);
+  std::vector<uint64_t> seeds(L);
+  for (int t = 0; t < L; ++t) {
+    const float* At = ht->A.data() + (size_t)t * D * M;
+    for (int m = 0; m < M; ++m) {
+      const int j = t * M + m;
+      double nrm = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const float a = At[(size_t)m * D + d];
+        Ad[(size_t)d * Jp + j] = a;
+        Aorig[(size_t)j * D + d] = a;
+        At64[((size_t)t * D + d) * M + m] = (double)a;
+        nrm += (double)a * (double)a;
+      }
+      bj[j] = ht->b[(size_t)t * M + m];
+      naj[j] = (float)(std::sqrt(nrm) * (1.0 + 0x1.0p-20));
+      b64[(size_t)t * M + m] = (double)ht->b[(size_t)t * M + m];
+    }
+    seeds[t] = mix64(ht->hash_seed + (uint64_t)t);
+  }
+  TRY(ht->dAd.exact(Ad.size()));
+  TRY(ht->dAorig.exact(Aorig.size()));
+  TRY(ht->dbj.exact(Jp));
+  TRY(ht->dnaj.exact(Jp));
+  TRY(ht->dAt64.exact(At64.size()));
+  TRY(ht->db64.exact(b64.size()));
+  TRY(ht->dseeds.exact(L));
+  HIP_TRY(hipMemcpy(ht->dAd.p, Ad.data(), Ad.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dAorig.p, Aorig.data(), Aorig.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dbj.p, bj.data(), Jp * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dnaj.p, naj.data(), Jp * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dAt64.p, At64.data(), At64.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->db64.p, b64.data(), b64.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dseeds.p, seeds.data(), L * 8, hipMemcpyHostToDevice));
+  return CULSH_OK;
+}
+
+int set_params(culsh_ht* ht, int N, int D, int L, int M, float W) {
+  ht->drop_index();
+  ht->has_params = false;
+  ht->N = N;
+  ht->D = D;
+  ht->L = L;
+  ht->M = M;
+  ht->W = W;
+  if (!params_ok(N, D, L, M, W))
+    return fail(CULSH_E_ARG,
+                "Projection parameters must be positive! Have you reset the index structure "
+                "properly???");
+  TRY(check_limits(N, D, L, M));
+  return CULSH_OK;
+}
+
+// Keys -> sort -> buckets -> directory, from codes [L][N][M] on the device.
+int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
+  hipStream_t s = ht->stream;
+  const long long N = ht->N, L = ht->L, M = ht->M, n_all = L * N;
+  ht->tb = std::max(1, ceil_log2_host(L));
+  ht->P = std::min(24, std::max(1, floor_log2_host(N) - 1));
+  ht->shift = 64 - ht->tb - ht->P;
+  ht->nslots = 1ll << (ht->tb + ht->P);
+
+  DevBuf<uint64_t> k0, k1;
+  DevBuf<uint32_t> v0, v1, flag, excl, scratch;
+  DevBuf<int> dflags;
+  TRY(k0.ensure(n_all));
+  TRY(k1.ensure(n_all));
+  TRY(v0.ensure(n_all));
+  TRY(v1.ensure(n_all));
+  TRY(flag.ensure(n_all));
+  TRY(excl.ensure(n_all));
+  TRY(scratch.ensure(std::max(rs_scratch_elems(n_all), scan_scratch_elems(n_all)) + 1));
+  TRY(dflags.ensure(1));
+  for (int attempt = 0;; ++attempt) {
+    if (attempt > 0) {
+      // 64-bit hash collision between two distinct codes: new seeds.
+      ht->hash_seed = mix64(ht->hash_seed + 0x9E3779B97F4A7C15ull);
+      std::vector<uint64_t> seeds(L);
+      for (int t = 0; t < L; ++t) seeds[t] = mix64(ht->hash_seed + (uint64_t)t);
+      HIP_TRY(hipMemcpy(ht->dseeds.p, seeds.data(), L * 8, hipMemcpyHostToDevice));
+    }
+    if (attempt > 4) return fail(CULSH_E_DEVICE, "bucket key collisions persist after reseeding");
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n_all)), dim3(256), 0, s, codes.p, N, (int)L,
+                       (int)M, (const uint64_t*)ht->dseeds.p, ht->tb, k0.p, v0.p);
+    uint64_t* kp = k0.p;
+    uint32_t* vp = v0.p;
+    uint64_t* ka = k1.p;
+    uint32_t* va = v1.p;
+    radix_sort_pairs(&kp, &vp, ka, va, n_all, 0, 64, scratch.p, s);
+    if (kp != k0.p) {
+      std::swap(k0.p, k1.p);
+      std::swap(v0.p, v1.p);
+    }
+    HIP_TRY(hipMemsetAsync(dflags.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_heads, dim3(grid_for(n_all)), dim3(256), 0, s, (const uint64_t*)k0.p,
+                       (const uint32_t*)v0.p, n_all, N, (int)M, ht->tb, codes.p, flag.p,
+                       dflags.p);
+    int collision = 0;
+    HIP_TRY(hipMemcpyAsync(&collision, dflags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!collision) break;
+    if (dbg) fprintf(dbg, "[HashTables::index]\tbucket key collision, reseeding\n");
+  }
+  scan_exclusive(flag.p, excl.p, n_all, scratch.p, s);
+  DevBuf<uint32_t> bases;
+  TRY(bases.ensure(L + 1));
+  hipLaunchKernelGGL(k_table_bases, dim3(1), dim3(std::max(64, (int)((L + 64) / 64 * 64))), 0, s,
+                     (const uint32_t*)flag.p, (const uint32_t*)excl.p, N, (int)L, bases.p);
+  ht->table_base.assign(L + 1, 0);
+  HIP_TRY(hipMemcpyAsync(ht->table_base.data(), bases.p, (L + 1) * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const long long B = ht->table_base[L];
+  TRY(ht->ukeys.exact(B));
+  TRY(ht->bstart.exact(B + 1));
+  TRY(ht->bcodes.exact(B * M));
+  hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n_all)), dim3(256), 0, s,
+                     (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
+                     (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->ukeys.p,
+                     ht->bstart.p, ht->bcodes.p);
+  const uint32_t endpos = (uint32_t)n_all;
+  HIP_TRY(hipMemcpyAsync(ht->bstart.p + B, &endpos, 4, hipMemcpyHostToDevice, s));
+  TRY(ht->dir.exact(ht->nslots + 1));
+  hipLaunchKernelGGL(k_directory, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, s,
+                     (const uint64_t*)ht->ukeys.p, B, ht->shift, ht->nslots, ht->dir.p);
+  // The sorted member ids become the index's id array.
+  ht->sids.release();
+  ht->sids.p = v0.p;
+  ht->sids.n = v0.n;
+  v0.p = nullptr;
+  v0.n = 0;
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  return CULSH_OK;
+}
+
+// Projection of the resident dataset into codes [L][N][M].
+int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
+  hipStream_t s = ht->stream;
+  const long long N = ht->N;
+  const int D = ht->D, M = ht->M, J = ht->J, Jp = ht->Jp;
+  TRY(codes.exact((size_t)N * J));
+  DevBuf<float> nx;
+  TRY(nx.ensure(N));
+  hipLaunchKernelGGL(k_row_norms, dim3(grid_for(N, 16)), dim3(256), 0, s, ht->dX, N, D, nx.p);
+  const unsigned long long cap = std::max<unsigned long long>(1ull << 20, (unsigned long long)(N * J / 64));
+  DevBuf<unsigned long long> list, cnt;
+  DevBuf<int> err;
+  TRY(list.ensure(cap));
+  TRY(cnt.ensure(1));
+  TRY(err.ensure(1));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, 8, s));
+  HIP_TRY(hipMemsetAsync(err.p, 0, 4, s));
+  const long long nrt = (N + kPjRows - 1) / kPjRows;
+  const long long tiles = nrt * (Jp / kPjCols);
+  const long long grid = (tiles + 7) / 8 * 8;
+  if (grid >= (1ll << 31)) return fail(CULSH_E_LIMIT, "projection grid too large");
+  hipLaunchKernelGGL(k_project_f32, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
+                     (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
+                     (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p, cnt.p,
+                     cap, err.p);
+  unsigned long long count = 0;
+  HIP_TRY(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  ht->rechecked = count;
+  if (count > cap) {
+    if (dbg) fprintf(dbg, "[HashTables::index]\t%llu boundary projections; full FP64 pass\n", count);
+    hipLaunchKernelGGL(k_project_f64_all, dim3(grid_for(N * J)), dim3(256), 0, s, ht->dX, N, D,
+                       (const float*)ht->dAorig.p, (const float*)ht->dbj.p, ht->W, M, J, codes.p,
+                       err.p);
+  } else if (count > 0) {
+    hipLaunchKernelGGL(k_recheck, dim3(grid_for((long long)count)), dim3(256), 0, s, ht->dX, N,
+                       D, (const float*)ht->dAorig.p, (const float*)ht->dbj.p, ht->W, M,
+                       (const unsigned long long*)list.p, count, codes.p, err.p);
+  }
+  int herr = 0;
+  HIP_TRY(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  if (herr)
+    return fail(CULSH_E_RANGE, "a hash code falls outside [-32766, 32766] (W too small for the data?)");
+  if (dbg) fprintf(dbg, "[HashTables::index]\tprojected %lld x %d, %llu FP64 rechecks\n", N, J, count);
+  return CULSH_OK;
+}
+
+int index_resident(culsh_ht* ht, FILE* dbg) {
+  DevBuf<int16_t> codes;
+  TRY(project_dataset(ht, codes, dbg));
+  TRY(build_buckets(ht, codes, dbg));
+  ht->indexed = true;
+  ht->canon = false;
+  if (dbg)
+    fprintf(dbg, "[HashTables::index]\t%d tables indexed, %lld buckets in total\n", ht->L,
+            ht->B_total());
+  return CULSH_OK;
+}
+
+// Lexicographic bucket numbering per table (the reference's bucket order,
+// cuLSH_Indexing.cu:71-93) and host mirrors of the bucket arrays.
+int ensure_canonical(culsh_ht* ht) {
+  if (ht->canon) return CULSH_OK;
+  if (!ht->indexed) return fail(CULSH_E_STATE, "no index: call index() or load() first");
+  const long long N = ht->N, L = ht->L, M = ht->M, B = ht->B_total();
+  std::vector<int16_t> codes((size_t)B * M);
+  std::vector<uint32_t> start(B + 1), ids((size_t)L * N);
+  HIP_TRY(hipStreamSynchronize(ht->stream));
+  HIP_TRY(hipMemcpy(codes.data(), ht->bcodes.p, codes.size() * 2, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(start.data(), ht->bstart.p, (B + 1) * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ids.data(), ht->sids.p, ids.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<int> renum(B);
+  ht->hB.assign(L, 0);
+  ht->hTotal.assign(L, {});
+  ht->hStart.assign(L, {});
+  ht->hSize.assign(L, {});
+  ht->hCodes.assign(L, {});
+  auto work = [&](int t) {
+    const long long b0 = ht->table_base[t], b1 = ht->table_base[t + 1], Bt = b1 - b0;
+    std::vector<uint32_t> order(Bt);
+    for (long long i = 0; i < Bt; ++i) order[i] = (uint32_t)(b0 + i);
+    const int16_t* cp = codes.data();
+    std::sort(order.begin(), order.end(), [cp, M](uint32_t a, uint32_t c) {
+      const int16_t* x = cp + (size_t)a * M;
+      const int16_t* y = cp + (size_t)c * M;
+      for (int m = 0; m < M; ++m)
+        if (x[m] != y[m]) return x[m] < y[m];
+      return false;
+    });
+    ht->hB[t] = (unsigned)Bt;
+    auto& tot = ht->hTotal[t];
+    auto& st = ht->hStart[t];
+    auto& sz = ht->hSize[t];
+    auto& cd = ht->hCodes[t];
+    tot.resize(N);
+    st.resize(Bt);
+    sz.resize(Bt);
+    cd.resize((size_t)Bt * M);
+    unsigned pos = 0;
+    for (long long i = 0; i < Bt; ++i) {
+      const uint32_t k = order[i];
+      renum[k] = (int)i;
+      const uint32_t a = start[k], e = start[k + 1];
+      st[i] = pos;
+      sz[i] = e - a;
+      for (uint32_t r = a; r < e; ++r) tot[pos++] = ids[r];
+      for (int m = 0; m < M; ++m) cd[(size_t)m * Bt + i] = (float)cp[(size_t)k * M + m];
+    }
+  };
+  {
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nth; ++w)
+      pool.emplace_back([&, w] {
+        for (int t = w; t < L; t += nth) work(t);
+      });
+    for (auto& th : pool) th.join();
+  }
+  TRY(ht->renumber.exact(std::max<long long>(B, 1)));
+  HIP_TRY(hipMemcpy(ht->renumber.p, renum.data(), B * 4, hipMemcpyHostToDevice));
+  ht->canon = true;
+  return CULSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* culsh_last_error(void) { return g_err.c_str(); }
+const char* culsh_version(void) { return "culsh-mi355x 0.1 (gfx950)"; }
+
+int culsh_ht_create(culsh_ht** out) {
+  if (!out) return fail(CULSH_E_ARG, "null output pointer");
+  *out = new culsh_ht;
+  return CULSH_OK;
+}
+
+void culsh_ht_destroy(culsh_ht* ht) { delete ht; }
+
+int culsh_ht_reset(culsh_ht* ht, int N, int D, int L, int M, float W, const float* const* A,
+                   const float* const* b, FILE* dbg) {
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  TRY(set_params(ht, N, D, L, M, W));
+  if (!A || !b) return fail(CULSH_E_ARG, "null A or b");
+  ht->A.resize((size_t)L * D * M);
+  ht->b.resize((size_t)L * M);
+  for (int t = 0; t < L; ++t) {
+    if (!A[t] || !b[t]) return fail(CULSH_E_ARG, "null A[%d] or b[%d]", t, t);
+    std::memcpy(ht->A.data() + (size_t)t * D * M, A[t], sizeof(float) * D * M);
+    std::memcpy(ht->b.data() + (size_t)t * M, b[t], sizeof(float) * M);
+  }
+  TRY(upload_params(ht));
+  ht->has_params = true;
+  if (dbg)
+    fprintf(dbg,
+            "[HashTables::reset]\tIndexing structure reset with parameters (N, D, L, M, W) = (%d, "
+            "%d, %d, %d, %.2f), matrices [A] & vectors [b] have been manually set.\n",
+            N, D, L, M, W);
+  return CULSH_OK;
+}
+
+int culsh_ht_reset_random(culsh_ht* ht, int N, int D, int L, int M, float W,
+                          unsigned long long seed, FILE* dbg) {
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  TRY(set_params(ht, N, D, L, M, W));
+  // cuLSH_HashTables.cu:122-187: A ~ N(0,1); b = (U - min U) * W, U ~ (0,1].
+  ht->A.resize((size_t)L * D * M);
+  ht->b.resize((size_t)L * M);
+  for (size_t i = 0; i < ht->A.size(); ++i) ht->A[i] = (float)culsh_rng::normal(seed, 101, i);
+  for (int t = 0; t < L; ++t) {
+    float* bt = ht->b.data() + (size_t)t * M;
+    for (int m = 0; m < M; ++m)
+      bt[m] = (float)culsh_rng::uniform_open0(seed, 102, (uint64_t)t * M + m);
+    float mn = bt[0];
+    for (int m = 1; m < M; ++m) mn = std::min(mn, bt[m]);
+    for (int m = 0; m < M; ++m) bt[m] = (bt[m] - mn) * W;
+  }
+  TRY(upload_params(ht));
+  ht->has_params = true;
+  if (dbg)
+    fprintf(dbg,
+            "[HashTables::reset]\tIndexing structure reset with parameters (N, D, L, M, W) = (%d, "
+            "%d, %d, %d, %.2f), and random matrices [A] & vectors [b].\n",
+            N, D, L, M, W);
+  return CULSH_OK;
+}
+
+static int index_common(culsh_ht* ht, const float* matrix, bool device, FILE* dbg) {
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  if (!params_ok(ht->N, ht->D, ht->L, ht->M, ht->W))
+    return fail(CULSH_E_STATE,
+                "Parameters must be positive. Have you reset the index structure properly???");
+  if (!ht->has_params)
+    return fail(CULSH_E_STATE,
+                "Pointers to projection matrices [A] and vectors [b] must be valid. Have you "
+                "reset the index structure properly???");
+  if (!matrix) return fail(CULSH_E_ARG, "null dataset");
+  TRY(ht_stream(ht));
+  ht->drop_index();
+  const size_t n = (size_t)ht->N * ht->D;
+  if (device) {
+    ht->dX = matrix;
+  } else {
+    TRY(ht->dX_own.exact(n));
+    HIP_TRY(hipMemcpy(ht->dX_own.p, matrix, n * 4, hipMemcpyHostToDevice));
+    ht->dX = ht->dX_own.p;
+    ht->hX_indexed = matrix;
+  }
+  int rc = index_resident(ht, dbg);
+  if (rc != CULSH_OK) {
+    std::string keep = g_err;
+    ht->drop_index();
+    g_err = keep;
+  }
+  return rc;
+}
+
+int culsh_ht_index(culsh_ht* ht, const float* matrix, FILE* dbg) {
+  return index_common(ht, matrix, false, dbg);
+}
+
+int culsh_ht_index_device(culsh_ht* ht, const float* d_matrix, FILE* dbg) {
+  return index_common(ht, d_matrix, true, dbg);
+}
+
+int culsh_ht_N(const culsh_ht* ht) { return ht ? ht->N : 0; }
+int culsh_ht_D(const culsh_ht* ht) { return ht ? ht->D : 0; }
+int culsh_ht_L(const culsh_ht* ht) { return ht ? ht->L : 0; }
+int culsh_ht_M(const culsh_ht* ht) { return ht ? ht->M : 0; }
+float culsh_ht_W(const culsh_ht* ht) { return ht ? ht->W : 0.f; }
+
+const float* culsh_ht_A(culsh_ht* ht, int t) {
+  if (!ht || !ht->has_params || t < 0 || t >= ht->L) return nullptr;
+  return ht->A.data() + (size_t)t * ht->D * ht->M;
+}
+const float* culsh_ht_b(culsh_ht* ht, int t) {
+  if (!ht || !ht->has_params || t < 0 || t >= ht->L) return nullptr;
+  return ht->b.data() + (size_t)t * ht->M;
+}
+
+unsigned culsh_ht_buckets(culsh_ht* ht, int t) {
+  if (!ht || !ht->indexed || t < 0 || t >= ht->L) return 0;
+  return ht->table_base[t + 1] - ht->table_base[t];
+}
+const unsigned* culsh_ht_buckets_all(culsh_ht* ht) {
+  if (!ht || ensure_canonical(ht) != CULSH_OK) return nullptr;
+  return ht->hB.data();
+}
+const unsigned* culsh_ht_buckets_total_indices(culsh_ht* ht, int t) {
+  if (!ht || t < 0 || t >= ht->L || ensure_canonical(ht) != CULSH_OK) return nullptr;
+  return ht->hTotal[t].data();
+}
+const unsigned* culsh_ht_buckets_starting_indices(culsh_ht* ht, int t) {
+  if (!ht || t < 0 || t >= ht->L || ensure_canonical(ht) != CULSH_OK) return nullptr;
+  return ht->hStart[t].data();
+}
+const unsigned* culsh_ht_buckets_sizes(culsh_ht* ht, int t) {
+  if (!ht || t < 0 || t >= ht->L || ensure_canonical(ht) != CULSH_OK) return nullptr;
+  return ht->hSize[t].data();
+}
+const float* culsh_ht_buckets_codes(culsh_ht* ht, int t) {
+  if (!ht || t < 0 || t >= ht->L || ensure_canonical(ht) != CULSH_OK) return nullptr;
+  return ht->hCodes[t].data();
+}
+unsigned long long culsh_ht_rechecked(const culsh_ht* ht) { return ht ? ht->rechecked : 0; }
+
+int culsh_ht_set_form(culsh_ht* ht, int form) {
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  if (form != CULSH_FORM_PLAIN) return fail(CULSH_E_LIMIT, "form %d not available", form);
+  ht->form = form;
+  return CULSH_OK;
+}
+
+int culsh_ht_save(culsh_ht* ht, const char* filename, FILE* dbg) {
+  (void)dbg;
+  if (!ht || !filename) return fail(CULSH_E_ARG, "null argument");
+  return fail(CULSH_E_LIMIT, "save is not available in this build");
+}
+
+int culsh_ht_load(culsh_ht* ht, const char* filename, FILE* dbg) {
+  (void)dbg;
+  if (!ht || !filename) return fail(CULSH_E_ARG, "null argument");
+  return fail(CULSH_E_LIMIT, "load is not available in this build");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// SearchTables
+// ---------------------------------------------------------------------------
+struct culsh_st {
+  culsh_ht* ht = nullptr;
+  int K = 0, T = 1, Q = 0;
+  int form = CULSH_FORM_PLAIN;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  DevBuf<float> dQ, dX_alt, ddists;
+  DevBuf<int> dqb, dids, dov, dqb_local;
+  DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt;
+  DevBuf<int> dnov;
+  std::vector<int> ids;
+  std::vector<float> dists;
+  std::vector<std::vector<int>> qbuckets;
+  std::vector<unsigned long long> cnu, cu;
+  unsigned long long tot_u = 0, tot_nu = 0, rechecked = 0;
+  culsh_times times{};
+  ~culsh_st() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void clear_results() {
+    ids.clear();
+    dists.clear();
+    qbuckets.clear();
+    cnu.clear();
+    cu.clear();
+    tot_u = tot_nu = rechecked = 0;
+    Q = 0;
+  }
+};
+
+namespace {
+
+int st_setup(culsh_st* st) {
+  if (!st->stream) HIP_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  for (auto& e : st->ev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  return CULSH_OK;
+}
+
+// The device search pipeline on `s`: probe kernel, candidate kernel.
+// Synchronises once to check the per-query capacity flags.
+int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
+               hipStream_t s, bool timed) {
+  culsh_ht* ht = st->ht;
+  const int L = ht->L, M = ht->M, D = ht->D, T = st->T, K = st->K;
+  const long long QT = (long long)Q * T;
+  if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
+  TRY(st->dqb.ensure((size_t)L * QT));
+  TRY(st->dcnu_qt.ensure((size_t)Q * L));
+  TRY(st->dcnu_q.ensure(Q));
+  TRY(st->dcu_q.ensure(Q));
+  TRY(st->dov.ensure(Q));
+  TRY(st->dnov.ensure(1));
+  TRY(st->dcnt.ensure(1));
+  HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
+  if (timed) HIP_TRY(hipEventRecord(st->ev[0], s));
+  const long long pairs = (long long)Q * L;
+  const dim3 pgrid((unsigned)((pairs + 3) / 4));
+#define CULSH_PROBE(JJ)                                                                       \
+  hipLaunchKernelGGL(k_query_probe<JJ>, pgrid, dim3(256), 0, s, dQ, Q, D, L, M, T,           \
+                     (const double*)ht->dAt64.p, (const double*)ht->db64.p, (double)ht->W,   \
+                     (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
+                     (const uint32_t*)ht->dir.p, (const uint64_t*)ht->ukeys.p,               \
+                     (const uint32_t*)ht->bstart.p, (const int16_t*)ht->bcodes.p, st->dqb.p, \
+                     st->dcnu_qt.p)
+  if (T <= 64)
+    CULSH_PROBE(1);
+  else if (T <= 128)
+    CULSH_PROBE(2);
+  else if (T <= 256)
+    CULSH_PROBE(4);
+  else if (T <= 512)
+    CULSH_PROBE(8);
+  else if (T <= 1024)
+    CULSH_PROBE(16);
+  else
+    return fail(CULSH_E_LIMIT, "T = %d exceeds the supported maximum of 1024", T);
+#undef CULSH_PROBE
+  if (timed) HIP_TRY(hipEventRecord(st->ev[1], s));
+  const double rel = 2.0 * (double)(D + 8) * 0x1.0p-24;
+  hipLaunchKernelGGL(k_candidates, dim3(Q), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
+                     (const int*)st->dqb.p, (const uint32_t*)ht->bstart.p,
+                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p, d_ids,
+                     d_dists, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p, rel, st->dcnt.p);
+  if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
+  HIP_TRY(hipGetLastError());
+  int nov = 0;
+  HIP_TRY(hipMemcpyAsync(&nov, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (nov > 0)
+    return fail(CULSH_E_LIMIT,
+                "%d queries exceed the per-query candidate capacity (%d unique, %d near the K-th "
+                "distance)",
+                nov, kUcap, kScap);
+  return CULSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int culsh_st_create(culsh_st** out) {
+  if (!out) return fail(CULSH_E_ARG, "null output pointer");
+  *out = new culsh_st;
+  return CULSH_OK;
+}
+
+void culsh_st_destroy(culsh_st* st) { delete st; }
+
+int culsh_st_reset(culsh_st* st, culsh_ht* ht, int K, int T) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  st->ht = ht;
+  st->K = K;
+  st->T = T;
+  st->clear_results();
+  return CULSH_OK;
+}
+
+int culsh_st_set_form(culsh_st* st, int form) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  if (form != CULSH_FORM_PLAIN) return fail(CULSH_E_LIMIT, "form %d not available", form);
+  st->form = form;
+  return CULSH_OK;
+}
+
+static int st_check(culsh_st* st, int Q) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  if (!st->ht)
+    return fail(CULSH_E_STATE,
+                "No indexing structure has been associated with searching structure...");
+  if (Q <= 0 || st->K <= 0 || st->T <= 0 || st->ht->L <= 0)
+    return fail(CULSH_E_ARG,
+                "Integer parameters must be all positive. Have you reset the indexing structure "
+                "properly before assigning it to this searching structure???");
+  if (!st->ht->indexed) return fail(CULSH_E_STATE, "the hash tables hold no index");
+  return st_setup(st);
+}
+
+int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* dataset, FILE* dbg) {
+  TRY(st_check(st, Q));
+  if (!queries || !dataset) return fail(CULSH_E_ARG, "null queries or dataset");
+  culsh_ht* ht = st->ht;
+  const int D = ht->D, K = st->K, T = st->T, L = ht->L;
+  st->clear_results();
+  TRY(ensure_canonical(ht));
+  const float* dX = ht->dX;
+  if (!dX || dataset != ht->hX_indexed) {
+    // A dataset other than the resident one: upload it for the distances.
+    TRY(st->dX_alt.ensure((size_t)ht->N * D));
+    HIP_TRY(hipMemcpy(st->dX_alt.p, dataset, (size_t)ht->N * D * 4, hipMemcpyHostToDevice));
+    dX = st->dX_alt.p;
+  }
+  TRY(st->dQ.ensure((size_t)Q * D));
+  TRY(st->dids.ensure((size_t)Q * K));
+  TRY(st->ddists.ensure((size_t)Q * K));
+  HIP_TRY(hipMemcpyAsync(st->dQ.p, queries, (size_t)Q * D * 4, hipMemcpyHostToDevice, st->stream));
+  if (dbg)
+    fprintf(dbg, "[SearchTables::search]\tPerforming searching. (Q, K, T) = (%d, %d, %d)\n", Q, K,
+            T);
+  TRY(run_search(st, st->dQ.p, dX, Q, st->dids.p, st->ddists.p, st->stream, true));
+  const long long QT = (long long)Q * T;
+  TRY(st->dqb_local.ensure((size_t)L * QT));
+  hipLaunchKernelGGL(k_local_buckets, dim3(grid_for(L * QT)), dim3(256), 0, st->stream,
+                     (const int*)st->dqb.p, QT, L, (const int*)ht->renumber.p, st->dqb_local.p);
+  st->Q = Q;
+  st->ids.resize((size_t)Q * K);
+  st->dists.resize((size_t)Q * K);
+  st->cnu.resize(Q);
+  st->cu.resize(Q);
+  st->qbuckets.assign(L, std::vector<int>(QT));
+  HIP_TRY(hipMemcpyAsync(st->ids.data(), st->dids.p, (size_t)Q * K * 4, hipMemcpyDeviceToHost, st->stream));
+  HIP_TRY(hipMemcpyAsync(st->dists.data(), st->ddists.p, (size_t)Q * K * 4, hipMemcpyDeviceToHost, st->stream));
+  HIP_TRY(hipMemcpyAsync(st->cnu.data(), st->dcnu_q.p, (size_t)Q * 8, hipMemcpyDeviceToHost, st->stream));
+  HIP_TRY(hipMemcpyAsync(st->cu.data(), st->dcu_q.p, (size_t)Q * 8, hipMemcpyDeviceToHost, st->stream));
+  for (int t = 0; t < L; ++t)
+    HIP_TRY(hipMemcpyAsync(st->qbuckets[t].data(), st->dqb_local.p + (size_t)t * QT, QT * 4,
+                           hipMemcpyDeviceToHost, st->stream));
+  unsigned long long nrc = 0;
+  HIP_TRY(hipMemcpyAsync(&nrc, st->dcnt.p, 8, hipMemcpyDeviceToHost, st->stream));
+  HIP_TRY(hipStreamSynchronize(st->stream));
+  st->rechecked = nrc;
+  for (int q = 0; q < Q; ++q) {
+    st->tot_nu += st->cnu[q];
+    st->tot_u += st->cu[q];
+  }
+  float ms1 = 0.f, ms2 = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms1, st->ev[0], st->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms2, st->ev[1], st->ev[2]));
+  st->times = culsh_times{};
+  st->times.buckets = ms1 / 1000.f;
+  st->times.distances = ms2 / 1000.f;
+  if (dbg)
+    fprintf(dbg, "Non-unique: %llu, unique: %llu, selectivity: %.2f%%\n", st->tot_nu, st->tot_u,
+            culsh_st_selectivity(st) * 100.0);
+  return CULSH_OK;
+}
+
+int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
+                           float* d_knn_dists, void* stream) {
+  TRY(st_check(st, Q));
+  if (!d_queries || !d_knn_ids || !d_knn_dists) return fail(CULSH_E_ARG, "null device pointer");
+  if (!st->ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
+  return run_search(st, d_queries, st->ht->dX, Q, d_knn_ids, d_knn_dists, (hipStream_t)stream,
+                    true);
+}
+
+int culsh_st_kernel_ms(culsh_st* st, float* probe_ms, float* candidates_ms) {
+  if (!st || !st->ev[2]) return fail(CULSH_E_STATE, "no search has run");
+  float a = 0.f, b = 0.f;
+  HIP_TRY(hipEventElapsedTime(&a, st->ev[0], st->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&b, st->ev[1], st->ev[2]));
+  if (probe_ms) *probe_ms = a;
+  if (candidates_ms) *candidates_ms = b;
+  return CULSH_OK;
+}
+
+const int* culsh_st_knn_ids(culsh_st* st) { return st && !st->ids.empty() ? st->ids.data() : nullptr; }
+const float* culsh_st_knn_dists(culsh_st* st) {
+  return st && !st->dists.empty() ? st->dists.data() : nullptr;
+}
+const int* culsh_st_query_buckets(culsh_st* st, int t) {
+  if (!st || t < 0 || t >= (int)st->qbuckets.size()) return nullptr;
+  return st->qbuckets[t].data();
+}
+float culsh_st_selectivity(culsh_st* st) {
+  if (!st || !st->ht || !st->ht->N || !st->Q) return 0.f;
+  return st->tot_u / (float)st->ht->N / (float)st->Q;
+}
+float culsh_st_nonunique_pct(culsh_st* st) {
+  if (!st || !st->tot_u) return 0.f;
+  return (st->tot_nu - st->tot_u) / (float)st->tot_u;
+}
+unsigned long long culsh_st_total_unique(culsh_st* st) { return st ? st->tot_u : 0; }
+unsigned long long culsh_st_total_nonunique(culsh_st* st) { return st ? st->tot_nu : 0; }
+void culsh_st_times(culsh_st* st, culsh_times* out) {
+  if (st && out) *out = st->times;
+}
+const unsigned long long* culsh_st_query_cnu(culsh_st* st) {
+  return st && !st->cnu.empty() ? st->cnu.data() : nullptr;
+}
+const unsigned long long* culsh_st_query_cu(culsh_st* st) {
+  return st && !st->cu.empty() ? st->cu.data() : nullptr;
+}
+unsigned long long culsh_st_rechecked(culsh_st* st) { return st ? st->rechecked : 0; }
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/*
+ * culsh_c.h — C ABI of the MI355X-native LSH index/query library (libculsh.so).
+ *
+ * Every entry point takes plain pointers and sizes.  Host pointers are read
+ * only during the call.  Status codes: 0 = success, non-zero = failure (see
+ * CULSH_E_*); culsh_last_error() returns the message of the calling thread's
+ * last failure.  Objects are not thread-safe; use one per thread.
+ *
+ * Layouts follow the reference (cuLSH README.md:9, cuLSH_Indexing.cu:43-51):
+ *   dataset / queries : column-major float [D x N] (vector n at n*D)
+ *   A[t]              : column-major float [D x M] ((d,m) at m*D + d)
+ *   b[t]              : float [M]
+ *   knn ids / dists   : [K x Q] ((k, q) at q*K + k), ascending (dist, id),
+ *                       squared L2, padded with (FLT_MAX, -1)
+ *   query buckets     : per table int [Q*T], entry q*T + p, -1 = no bucket
+ *   bucket codes      : float column-major [B x M] ((k, m) at m*B + k)
+ * Bucket numbering and bucket arrays are in lexicographic code order, as the
+ * reference produces them (cuLSH_Indexing.cu:71-93).
+ */
+#ifndef CULSH_C_H
+#define CULSH_C_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct culsh_ht culsh_ht; /* cuLSH::HashTables  (cuLSH_HashTables.cu:51-108) */
+typedef struct culsh_st culsh_st; /* cuLSH::SearchTables (cuLSH_SearchTables.cu:42-107) */
+
+/* cuLSH::SearchTables::timesStruct (cuLSH_SearchTables.cu:44-52), seconds.
+ * collect = gathering bucket members, sort = ordering, unique = dedup; the
+ * device pipeline fuses collect/sort/unique, reported under collect. */
+typedef struct {
+  float buckets, collect, sort, unique, distances, knn;
+} culsh_times;
+
+enum {
+  CULSH_OK = 0,
+  CULSH_E_ARG = 1,     /* invalid argument or parameters not positive */
+  CULSH_E_STATE = 2,   /* call out of order (e.g. index before reset) */
+  CULSH_E_DEVICE = 3,  /* HIP runtime error */
+  CULSH_E_MEMORY = 4,  /* allocation failed / does not fit */
+  CULSH_E_IO = 5,      /* file open/read/write failed */
+  CULSH_E_RANGE = 6,   /* a hash code does not fit the 16-bit code store */
+  CULSH_E_LIMIT = 7    /* a size limit of this implementation (M > 64, ...) */
+};
+
+/* Search forms (see culsh_st_set_form). */
+enum {
+  CULSH_FORM_PLAIN = 0,     /* FP32 projection (MFMA), direct FP32 distances */
+  CULSH_FORM_PRECISION = 1  /* BF16x3 projection, norm-expansion distances */
+};
+
+const char* culsh_last_error(void);
+const char* culsh_version(void);
+
+/* ---- HashTables -------------------------------------------------------- */
+int culsh_ht_create(culsh_ht** out);
+void culsh_ht_destroy(culsh_ht* ht);
+
+/* HashTables::reset(N,D,L,M,W,dbg) (cuLSH_HashTables.cu:190-210): random
+ * A ~ N(0,1), b = (U - min U) * W per table.  The reference seeds with
+ * time(0); this entry point takes the seed explicitly. */
+int culsh_ht_reset_random(culsh_ht* ht, int N, int D, int L, int M, float W,
+                          unsigned long long seed, FILE* dbg);
+/* HashTables::reset(N,D,L,M,W,A,b,dbg) (cuLSH_HashTables.cu:213-241):
+ * deep-copies A[t] ([D x M]) and b[t] ([M]) for t < L. */
+int culsh_ht_reset(culsh_ht* ht, int N, int D, int L, int M, float W,
+                   const float* const* A, const float* const* b, FILE* dbg);
+/* HashTables::reset(filename, dbg) / load (cuLSH_HashTables.cu:243-249,441-483). */
+int culsh_ht_load(culsh_ht* ht, const char* filename, FILE* dbg);
+/* HashTables::save (cuLSH_HashTables.cu:395-438), byte-compatible format. */
+int culsh_ht_save(culsh_ht* ht, const char* filename, FILE* dbg);
+
+/* HashTables::index(matrix, dbg) (cuLSH_HashTables.cu:486-520): matrix is a
+ * host [D x N] dataset.  The dataset stays resident on the device. */
+int culsh_ht_index(culsh_ht* ht, const float* matrix, FILE* dbg);
+/* Same, from a device pointer (dataset already in HBM, not copied: the
+ * caller keeps it alive while the index is used). */
+int culsh_ht_index_device(culsh_ht* ht, const float* d_matrix, FILE* dbg);
+
+/* Getters (cuLSH_HashTables.cu:89-103).  Bucket arrays are host mirrors,
+ * materialised on first use, valid until the next reset/index/load. */
+int culsh_ht_N(const culsh_ht* ht);
+int culsh_ht_D(const culsh_ht* ht);
+int culsh_ht_L(const culsh_ht* ht);
+int culsh_ht_M(const culsh_ht* ht);
+float culsh_ht_W(const culsh_ht* ht);
+const float* culsh_ht_A(culsh_ht* ht, int table);
+const float* culsh_ht_b(culsh_ht* ht, int table);
+unsigned culsh_ht_buckets(culsh_ht* ht, int table);
+const unsigned* culsh_ht_buckets_all(culsh_ht* ht);
+const unsigned* culsh_ht_buckets_total_indices(culsh_ht* ht, int table);
+const unsigned* culsh_ht_buckets_starting_indices(culsh_ht* ht, int table);
+const unsigned* culsh_ht_buckets_sizes(culsh_ht* ht, int table);
+const float* culsh_ht_buckets_codes(culsh_ht* ht, int table);
+/* Number of data projections re-evaluated in FP64 by the last index() (their
+ * FP32/BF16x3 value was within the rounding-error bound of a bucket edge). */
+unsigned long long culsh_ht_rechecked(const culsh_ht* ht);
+/* Projection form of index(): CULSH_FORM_PLAIN (default) or _PRECISION. */
+int culsh_ht_set_form(culsh_ht* ht, int form);
+
+/* ---- SearchTables ------------------------------------------------------ */
+int culsh_st_create(culsh_st** out);
+void culsh_st_destroy(culsh_st* st);
+/* SearchTables::reset(ht, K, T) (cuLSH_SearchTables.cu:127-139). */
+int culsh_st_reset(culsh_st* st, culsh_ht* ht, int K, int T);
+/* SearchTables::search(queries, Q, dataset, dbg) (cuLSH_SearchTables.cu:197-463):
+ * host [D x Q] queries; dataset must be the indexed one (when it is the same
+ * host pointer index() was given, the resident copy is used). */
+int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* dataset, FILE* dbg);
+/* Device-resident search: queries, ids and dists are device pointers
+ * ([D x Q], [K x Q], [K x Q]); runs on `stream` (a hipStream_t, 0 = null
+ * stream) and returns without synchronising.  Query buckets and candidate
+ * totals of this call are not copied to the host. */
+int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
+                           float* d_knn_dists, void* stream);
+/* Distance form of search(): CULSH_FORM_PLAIN (default) or _PRECISION. */
+int culsh_st_set_form(culsh_st* st, int form);
+
+/* Results and metrics (cuLSH_SearchTables.cu:83-105). */
+const int* culsh_st_knn_ids(culsh_st* st);
+const float* culsh_st_knn_dists(culsh_st* st);
+const int* culsh_st_query_buckets(culsh_st* st, int table);
+float culsh_st_selectivity(culsh_st* st);
+float culsh_st_nonunique_pct(culsh_st* st);
+unsigned long long culsh_st_total_unique(culsh_st* st);
+unsigned long long culsh_st_total_nonunique(culsh_st* st);
+void culsh_st_times(culsh_st* st, culsh_times* out);
+/* Per-query candidate counts of the last search (host pointers, [Q]). */
+const unsigned long long* culsh_st_query_cnu(culsh_st* st);
+const unsigned long long* culsh_st_query_cu(culsh_st* st);
+/* Candidates re-evaluated in FP64 by the last search (distance boundary). */
+unsigned long long culsh_st_rechecked(culsh_st* st);
+/* Device time (ms, HIP events on the search stream) of the last search's
+ * probe kernel (projection + multiprobe + lookup) and candidate kernel
+ * (gather + dedup + distances + top-K). */
+int culsh_st_kernel_ms(culsh_st* st, float* probe_ms, float* candidates_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CULSH_C_H */

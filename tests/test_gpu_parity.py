@@ -1,0 +1,161 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (through the C ABI).
+
+Bar: bit-exact ids, distances ((float) of the FP64 distance), per-query
+candidate counts, query buckets and bucket arrays (lexicographic numbering,
+as the reference produces them)."""
+import numpy as np
+import pytest
+
+from culsh_amd import workload as wl
+from lshtest import assert_buckets_equal, assert_search_equal, run_gpu, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(seed, N=2000, D=16, L=4, M=6, W=1.0, Q=50, centers=20, sigma=0.2):
+    X = wl.mixture(N, D, centers, sigma, seed)
+    Y = wl.queries_near(X, Q, 0.05, seed + 1)
+    A, b = wl.projection(L, D, M, W, seed + 2)
+    return X, Y, A, b
+
+
+@pytest.mark.parametrize("T", [1, 8, 40])
+@pytest.mark.parametrize("K", [1, 5, 33])
+def test_small_mixture(T, K):
+    print(f"small mixture T={T} K={K}", flush=True)
+    X, Y, A, b = _small(11)
+    g = run_gpu(X, Y, A, b, 4, 6, 1.0, K, T)
+    o = run_oracle(X, Y, A, b, 4, 6, 1.0, K, T)
+    assert_buckets_equal(g["ht"], o["index"], 4)
+    assert_search_equal(g, o)
+
+
+def test_tie_heavy_integers():
+    """Small integer data: many exact distance ties at the K-th boundary, so the
+    reference's tie rule decides which ids are returned."""
+    print("tie-heavy integers", flush=True)
+    rng = np.random.default_rng(5)
+    N, D, L, M, W, Q = 3000, 8, 6, 4, 3.0, 200
+    X = rng.integers(0, 4, size=(N, D)).astype(np.float32)
+    Y = rng.integers(0, 4, size=(Q, D)).astype(np.float32)
+    A, b = wl.projection(L, D, M, W, 77)
+    for K, T in [(10, 1), (16, 12), (3, 64), (50, 5)]:
+        g = run_gpu(X, Y, A, b, L, M, W, K, T)
+        o = run_oracle(X, Y, A, b, L, M, W, K, T)
+        assert_search_equal(g, o)
+
+
+def test_wide_probe_queue_and_many_dims():
+    """T > 64 (two queue slots per lane), M > 32 (64-delta truncation), D not a
+    multiple of 4."""
+    print("wide queue", flush=True)
+    X = wl.mixture(4000, 37, 10, 0.3, 21)
+    Y = wl.queries_near(X, 40, 0.05, 22)
+    L, M, W = 3, 40, 2.0
+    A, b = wl.projection(L, 37, M, W, 23)
+    for K, T in [(7, 100), (4, 65)]:
+        g = run_gpu(X, Y, A, b, L, M, W, K, T)
+        o = run_oracle(X, Y, A, b, L, M, W, K, T)
+        assert_buckets_equal(g["ht"], o["index"], L)
+        assert_search_equal(g, o)
+
+
+def test_exact_delta_ties():
+    """Duplicated projection columns give exactly tied deltas: the delta order
+    then follows the reference heap's mechanics."""
+    print("delta ties", flush=True)
+    X = wl.mixture(3000, 12, 8, 0.3, 31)
+    Y = wl.queries_near(X, 60, 0.05, 32)
+    L, M, W = 2, 6, 1.5
+    A, b = wl.projection(L, 12, M, W, 33)
+    A[:, 3] = A[:, 1]
+    b[:, 3] = b[:, 1]
+    A[:, 5] = A[:, 0]
+    b[:, 5] = b[:, 0]
+    g = run_gpu(X, Y, A, b, L, M, W, 6, 30)
+    o = run_oracle(X, Y, A, b, L, M, W, 6, 30)
+    assert_search_equal(g, o)
+
+
+def test_config1_full():
+    """c1: N=100k, D=128, L=8, M=16, W=4, Q=1k, K=10, T=1 -- ids exact."""
+    cfg = wl.CONFIGS["c1"]
+    print(cfg.describe(), flush=True)
+    X, Y, A, b = wl.make_data(cfg)
+    g = run_gpu(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    o = run_oracle(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    assert_buckets_equal(g["ht"], o["index"], cfg.L)
+    assert_search_equal(g, o)
+
+
+def test_config2_full():
+    """c2: N=1M, D=128 integer data, L=20, M=24, W=400, Q=10k, K=10, T=32."""
+    cfg = wl.CONFIGS["c2"]
+    print(cfg.describe(), flush=True)
+    X, Y, A, b = wl.make_data(cfg)
+    g = run_gpu(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    o = run_oracle(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    assert_search_equal(g, o)
+
+
+def test_config3_slice():
+    """c3 parameters (D=96, L=32, M=20, W=2, K=100, T=64) on a 400k-point,
+    1000-query slice of its mixture."""
+    cfg = wl.CONFIGS["c3"]
+    print(cfg.describe(), "slice N=400000 Q=1000", flush=True)
+    X, Y, A, b = wl.make_data(cfg, N=400_000, Q=1_000)
+    g = run_gpu(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    o = run_oracle(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
+    assert_search_equal(g, o)
+
+
+def test_device_api_matches_host_api():
+    import torch
+    print("device api", flush=True)
+    X, Y, A, b = _small(41, N=5000, Q=300)
+    g = run_gpu(X, Y, A, b, 4, 6, 1.0, 9, 16, want_buckets=False)
+    ht, st = g["ht"], g["st"]
+    dY = torch.from_numpy(Y).cuda()
+    ids = torch.empty((Y.shape[0], 9), dtype=torch.int32, device="cuda")
+    d = torch.empty((Y.shape[0], 9), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert st.search_device(dY, ids, d), st.last_error
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy(), g["ids"])
+    np.testing.assert_array_equal(d.cpu().numpy(), g["dists"])
+
+
+def test_index_device_resident_dataset():
+    import torch
+    print("index from device", flush=True)
+    from culsh_amd import HashTables, SearchTables
+    X, Y, A, b = _small(51, N=3000, Q=80)
+    o = run_oracle(X, Y, A, b, 4, 6, 1.0, 5, 8)
+    dX = torch.from_numpy(X).cuda()
+    ht = HashTables()
+    assert ht.reset(X.shape[0], X.shape[1], 4, 6, 1.0, A, b)
+    assert ht.index_device(dX), ht.last_error
+    st = SearchTables()
+    assert st.reset(ht, 5, 8)
+    assert st.search(Y, Y.shape[0], X), st.last_error
+    np.testing.assert_array_equal(st.getKnnIds(), o["ids"])
+
+
+def test_random_reset_statistics():
+    """reset(N,D,L,M,W) draws A ~ N(0,1) and b = (U - min U) W
+    (cuLSH_HashTables.cu:122-187): one b per table is exactly 0, all in [0, W)."""
+    from culsh_amd import HashTables, SearchTables
+    print("random reset", flush=True)
+    X = wl.mixture(2000, 16, 10, 0.2, 61)
+    ht = HashTables()
+    assert ht.reset(2000, 16, 5, 6, 2.5, seed=1234)
+    A = np.stack([ht.getA(t) for t in range(5)])
+    bb = np.stack([ht.getb(t) for t in range(5)])
+    assert abs(A.mean()) < 0.1 and abs(A.std() - 1) < 0.1
+    assert (bb.min(axis=1) == 0).all() and (bb < 2.5).all() and (bb >= 0).all()
+    assert ht.index(X)
+    st = SearchTables()
+    assert st.reset(ht, 4, 4)
+    assert st.search(X[:50], 50, X)
+    # A data point used as a query finds itself at distance 0.
+    assert (st.getKnnIds()[:, 0] == np.arange(50)).all()
