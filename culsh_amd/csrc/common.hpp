@@ -68,22 +68,118 @@ __device__ inline void wave_lds_sync() {
 
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Lower bound of `key` in the sorted unique key array, through the prefix
-// directory: dir[v] = first bucket whose key >> shift >= v.  Returns the
-// bucket index, or -1 when the key is absent.
-__device__ inline long long lookup_key(uint64_t key, const uint32_t* __restrict__ dir, int shift,
-                                       const uint64_t* __restrict__ ukeys) {
-  uint64_t v = key >> shift;
-  uint32_t lo = dir[v], hi = dir[v + 1];
+// ---- DPP cross-lane moves inside groups of 8 lanes (gfx9 DPP controls) ----
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror (i <-> 7-i).
+constexpr int kDppXor1 = 0xB1;
+constexpr int kDppXor2 = 0x4E;
+constexpr int kDppHalfMirror = 0x141;
+
+template <int CTRL>
+__device__ inline uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Reductions over the 8 lanes of an octet; every lane gets the result.  All
+// 64 lanes of the wave must execute them.
+__device__ inline uint64_t octet_min_u64(uint64_t v) {
+  uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v;
+#define CULSH_OCT_MIN64(C)                                \
+  {                                                       \
+    const uint32_t h2 = dpp_u32<C>(hi), l2 = dpp_u32<C>(lo); \
+    const bool lt = (h2 < hi) || (h2 == hi && l2 < lo);   \
+    hi = lt ? h2 : hi;                                    \
+    lo = lt ? l2 : lo;                                    \
+  }
+  CULSH_OCT_MIN64(kDppXor1)
+  CULSH_OCT_MIN64(kDppXor2)
+  CULSH_OCT_MIN64(kDppHalfMirror)
+#undef CULSH_OCT_MIN64
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t octet_min_u32(uint32_t v) {
+  v = min(v, dpp_u32<kDppXor1>(v));
+  v = min(v, dpp_u32<kDppXor2>(v));
+  v = min(v, dpp_u32<kDppHalfMirror>(v));
+  return v;
+}
+
+__device__ inline uint64_t octet_or_u64(uint64_t v) {
+  uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v;
+  hi |= dpp_u32<kDppXor1>(hi);
+  lo |= dpp_u32<kDppXor1>(lo);
+  hi |= dpp_u32<kDppXor2>(hi);
+  lo |= dpp_u32<kDppXor2>(lo);
+  hi |= dpp_u32<kDppHalfMirror>(hi);
+  lo |= dpp_u32<kDppHalfMirror>(lo);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint64_t octet_sum_u64(uint64_t v) {
+  uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v;
+#define CULSH_OCT_ADD64(C)                                  \
+  {                                                         \
+    const uint32_t h2 = dpp_u32<C>(hi), l2 = dpp_u32<C>(lo); \
+    const uint64_t s = (((uint64_t)hi << 32) | lo) + (((uint64_t)h2 << 32) | l2); \
+    hi = (uint32_t)(s >> 32);                               \
+    lo = (uint32_t)s;                                       \
+  }
+  CULSH_OCT_ADD64(kDppXor1)
+  CULSH_OCT_ADD64(kDppXor2)
+  CULSH_OCT_ADD64(kDppHalfMirror)
+#undef CULSH_OCT_ADD64
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ---- Bucket records and the key directory ----
+// One record per bucket: its key, the start of its members in the id array
+// and its size; a lookup hit then needs a single 16-byte load.
+struct __align__(16) BucketRec {
+  uint64_t key;
+  uint32_t start;
+  uint32_t size;
+};
+
+// Directory slot v covers keys with key >> shift == v.  Packed u64:
+//   bits  0-31  index of the slot's first record
+//   bits 32-55  low 24 bits of that record's key (fingerprint)
+//   bits 56-63  number of records in the slot, saturated at 255
+constexpr uint64_t kFpMask = 0xFFFFFFull;
+
+__device__ __host__ inline uint64_t dir_entry(uint32_t first, uint64_t first_key, uint32_t count) {
+  const uint64_t c = count > 255u ? 255u : count;
+  return (uint64_t)first | ((first_key & kFpMask) << 32) | (c << 56);
+}
+
+// Record index of `key`, or -1, given its directory entry e = dir[key >> shift].
+// Most absent keys are rejected by the entry alone (empty slot or
+// fingerprint mismatch).
+__device__ inline long long lookup_resolve(uint64_t key, uint64_t e,
+                                           const uint64_t* __restrict__ dir, int shift,
+                                           const BucketRec* __restrict__ recs) {
+  const uint64_t v = key >> shift;
+  const uint32_t cnt = (uint32_t)(e >> 56);
+  if (cnt == 0) return -1;
+  uint32_t lo = (uint32_t)e;
+  if (cnt == 1) {
+    if (((e >> 32) & kFpMask) != (key & kFpMask)) return -1;
+    return recs[lo].key == key ? (long long)lo : -1;
+  }
+  uint32_t hi = cnt < 255 ? lo + cnt : (uint32_t)dir[v + 1];
   const uint32_t end = hi;
   while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (ukeys[mid] < key)
+    const uint32_t mid = (lo + hi) >> 1;
+    if (recs[mid].key < key)
       lo = mid + 1;
     else
       hi = mid;
   }
-  return (lo < end && ukeys[lo] == key) ? (long long)lo : -1;
+  return (lo < end && recs[lo].key == key) ? (long long)lo : -1;
+}
+
+__device__ inline long long lookup_key(uint64_t key, const uint64_t* __restrict__ dir, int shift,
+                                       const BucketRec* __restrict__ recs) {
+  return lookup_resolve(key, dir[key >> shift], dir, shift, recs);
 }
 
 }  // namespace culsh

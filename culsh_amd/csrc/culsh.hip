@@ -135,11 +135,10 @@ struct culsh_ht {
   DevBuf<float> dX_own;
   const float* dX = nullptr;          // resident dataset [N][D]
   const float* hX_indexed = nullptr;  // host pointer index() was given
-  DevBuf<uint64_t> ukeys;             // [B_total] sorted unique keys
-  DevBuf<uint32_t> bstart;            // [B_total + 1] offsets into sids
+  DevBuf<BucketRec> recs;             // [B_total] sorted by key: key, start, size
   DevBuf<uint32_t> sids;              // [L*N] members, bucket by bucket
   DevBuf<int16_t> bcodes;             // [B_total][M]
-  DevBuf<uint32_t> dir;               // [nslots + 1]
+  DevBuf<uint64_t> dir;               // [nslots + 1] packed directory entries
   std::vector<uint32_t> table_base;   // [L + 1] first global bucket of each table
   int tb = 1, P = 1, shift = 0;
   long long nslots = 0;
@@ -162,8 +161,7 @@ struct culsh_ht {
     dX_own.release();
     dX = nullptr;
     hX_indexed = nullptr;
-    ukeys.release();
-    bstart.release();
+    recs.release();
     sids.release();
     bcodes.release();
     dir.release();
@@ -205,7 +203,10 @@ int upload_params(culsh_ht* ht) {
   ht->J = L * M;
   ht->Jp = (ht->J + kPjCols - 1) / kPjCols * kPjCols;
   const int Jp = ht->Jp;
-  std::vector<float> Ad((size_t)D * Jp, 0.f), Aorig((size_t)ht->J * D), bj(Jp, 0.f), naj(Jp, 0.f);
+  // Ad carries 64 floats of padding after its last row (the probe kernel's
+  // lanes past M read it and discard it).
+  std::vector<float> Ad((size_t)D * Jp + 64, 0.f), Aorig((size_t)ht->J * D), bj(Jp, 0.f),
+      naj(Jp, 0.f);
   std::vector<double> At64((size_t)L * D * M), b64((size_t)L * M);
   std::vector<uint64_t> seeds(L);
   for (int t = 0; t < L; ++t) {
@@ -259,14 +260,29 @@ int set_params(culsh_ht* ht, int N, int D, int L, int M, float W) {
   return CULSH_OK;
 }
 
+// Key directory over the sorted bucket records: 2^P slots per table with
+// 2^P >= 2 * (largest table's bucket count), so most slots hold at most one
+// key and an absent key is usually rejected by one 8-byte load.
+int build_directory(culsh_ht* ht) {
+  long long maxB = 1;
+  for (int t = 0; t < ht->L; ++t)
+    maxB = std::max<long long>(maxB, ht->table_base[t + 1] - ht->table_base[t]);
+  ht->P = std::max(1, ceil_log2_host(maxB) + 1);
+  ht->P = std::min(ht->P, 30 - ht->tb);  // at most 2^30 slots (8 GiB)
+  ht->shift = 64 - ht->tb - ht->P;
+  ht->nslots = 1ll << (ht->tb + ht->P);
+  TRY(ht->dir.exact(ht->nslots + 1));
+  hipLaunchKernelGGL(k_directory, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, ht->stream,
+                     (const BucketRec*)ht->recs.p, ht->B_total(), ht->shift, ht->nslots,
+                     ht->dir.p);
+  return CULSH_OK;
+}
+
 // Keys -> sort -> buckets -> directory, from codes [L][N][M] on the device.
 int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
   hipStream_t s = ht->stream;
   const long long N = ht->N, L = ht->L, M = ht->M, n_all = L * N;
   ht->tb = std::max(1, ceil_log2_host(L));
-  ht->P = std::min(24, std::max(1, floor_log2_host(N) - 1));
-  ht->shift = 64 - ht->tb - ht->P;
-  ht->nslots = 1ll << (ht->tb + ht->P);
 
   DevBuf<uint64_t> k0, k1;
   DevBuf<uint32_t> v0, v1, flag, excl, scratch;
@@ -318,18 +334,14 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
   HIP_TRY(hipMemcpyAsync(ht->table_base.data(), bases.p, (L + 1) * 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   const long long B = ht->table_base[L];
-  TRY(ht->ukeys.exact(B));
-  TRY(ht->bstart.exact(B + 1));
+  TRY(ht->recs.exact(B));
   TRY(ht->bcodes.exact(B * M));
   hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n_all)), dim3(256), 0, s,
                      (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
-                     (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->ukeys.p,
-                     ht->bstart.p, ht->bcodes.p);
-  const uint32_t endpos = (uint32_t)n_all;
-  HIP_TRY(hipMemcpyAsync(ht->bstart.p + B, &endpos, 4, hipMemcpyHostToDevice, s));
-  TRY(ht->dir.exact(ht->nslots + 1));
-  hipLaunchKernelGGL(k_directory, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, s,
-                     (const uint64_t*)ht->ukeys.p, B, ht->shift, ht->nslots, ht->dir.p);
+                     (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->recs.p,
+                     ht->bcodes.p);
+  hipLaunchKernelGGL(k_rec_sizes, dim3(grid_for(B)), dim3(256), 0, s, ht->recs.p, B, n_all);
+  TRY(build_directory(ht));
   // The sorted member ids become the index's id array.
   ht->sids.release();
   ht->sids.p = v0.p;
@@ -410,10 +422,11 @@ int ensure_canonical(culsh_ht* ht) {
   if (!ht->indexed) return fail(CULSH_E_STATE, "no index: call index() or load() first");
   const long long N = ht->N, L = ht->L, M = ht->M, B = ht->B_total();
   std::vector<int16_t> codes((size_t)B * M);
-  std::vector<uint32_t> start(B + 1), ids((size_t)L * N);
+  std::vector<BucketRec> recs(B);
+  std::vector<uint32_t> ids((size_t)L * N);
   HIP_TRY(hipStreamSynchronize(ht->stream));
   HIP_TRY(hipMemcpy(codes.data(), ht->bcodes.p, codes.size() * 2, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(start.data(), ht->bstart.p, (B + 1) * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(recs.data(), ht->recs.p, B * sizeof(BucketRec), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(ids.data(), ht->sids.p, ids.size() * 4, hipMemcpyDeviceToHost));
   std::vector<int> renum(B);
   ht->hB.assign(L, 0);
@@ -446,7 +459,7 @@ int ensure_canonical(culsh_ht* ht) {
     for (long long i = 0; i < Bt; ++i) {
       const uint32_t k = order[i];
       renum[k] = (int)i;
-      const uint32_t a = start[k], e = start[k + 1];
+      const uint32_t a = recs[k].start, e = a + recs[k].size;
       st[i] = pos;
       sz[i] = e - a;
       for (uint32_t r = a; r < e; ++r) tot[pos++] = ids[r];
@@ -694,31 +707,65 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
   if (timed) HIP_TRY(hipEventRecord(st->ev[0], s));
   const long long pairs = (long long)Q * L;
-  const dim3 pgrid((unsigned)((pairs + 3) / 4));
+  const int lds8 = probe8_lds_bytes(D, M);
+  if (T <= 128 && lds8 <= 64 * 1024) {
+    // Octet kernel: 32 queries of one table per workgroup.
+    const dim3 g8((unsigned)(((Q + 31) / 32) * (long long)L));
+#define CULSH_PROBE8(JJ, MM)                                                                  \
+  hipLaunchKernelGGL((k_query_probe8<JJ, MM>), g8, dim3(256), lds8, s, dQ, Q, D, L, M, T,     \
+                     (const float*)ht->dAd.p, ht->Jp, (const double*)ht->db64.p, (double)ht->W, \
+                     (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
+                     (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
+                     (const int16_t*)ht->bcodes.p, st->dqb.p, st->dcnu_qt.p)
+#define CULSH_PROBE8_M(JJ)   \
+  if (M <= 8)                \
+    CULSH_PROBE8(JJ, 1);     \
+  else if (M <= 16)          \
+    CULSH_PROBE8(JJ, 2);     \
+  else if (M <= 32)          \
+    CULSH_PROBE8(JJ, 4);     \
+  else                       \
+    CULSH_PROBE8(JJ, 8);
+    if (T <= 8) {
+      CULSH_PROBE8_M(1)
+    } else if (T <= 16) {
+      CULSH_PROBE8_M(2)
+    } else if (T <= 32) {
+      CULSH_PROBE8_M(4)
+    } else if (T <= 64) {
+      CULSH_PROBE8_M(8)
+    } else {
+      CULSH_PROBE8_M(16)
+    }
+#undef CULSH_PROBE8_M
+#undef CULSH_PROBE8
+  } else {
+    // Wave kernel: one wave per (query, table).
+    const dim3 pgrid((unsigned)((pairs + 3) / 4));
 #define CULSH_PROBE(JJ)                                                                       \
   hipLaunchKernelGGL(k_query_probe<JJ>, pgrid, dim3(256), 0, s, dQ, Q, D, L, M, T,           \
                      (const double*)ht->dAt64.p, (const double*)ht->db64.p, (double)ht->W,   \
                      (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
-                     (const uint32_t*)ht->dir.p, (const uint64_t*)ht->ukeys.p,               \
-                     (const uint32_t*)ht->bstart.p, (const int16_t*)ht->bcodes.p, st->dqb.p, \
-                     st->dcnu_qt.p)
-  if (T <= 64)
-    CULSH_PROBE(1);
-  else if (T <= 128)
-    CULSH_PROBE(2);
-  else if (T <= 256)
-    CULSH_PROBE(4);
-  else if (T <= 512)
-    CULSH_PROBE(8);
-  else if (T <= 1024)
-    CULSH_PROBE(16);
-  else
-    return fail(CULSH_E_LIMIT, "T = %d exceeds the supported maximum of 1024", T);
+                     (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
+                     (const int16_t*)ht->bcodes.p, st->dqb.p, st->dcnu_qt.p)
+    if (T <= 64)
+      CULSH_PROBE(1);
+    else if (T <= 128)
+      CULSH_PROBE(2);
+    else if (T <= 256)
+      CULSH_PROBE(4);
+    else if (T <= 512)
+      CULSH_PROBE(8);
+    else if (T <= 1024)
+      CULSH_PROBE(16);
+    else
+      return fail(CULSH_E_LIMIT, "T = %d exceeds the supported maximum of 1024", T);
 #undef CULSH_PROBE
+  }
   if (timed) HIP_TRY(hipEventRecord(st->ev[1], s));
   const double rel = 2.0 * (double)(D + 8) * 0x1.0p-24;
   hipLaunchKernelGGL(k_candidates, dim3(Q), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
-                     (const int*)st->dqb.p, (const uint32_t*)ht->bstart.p,
+                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
                      (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p, d_ids,
                      d_dists, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p, rel, st->dcnt.p);
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));

@@ -239,22 +239,45 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
   flag[i] = head ? 1u : 0u;
 }
 
-// Bucket arrays from heads: unique keys, start offsets into the sorted ids,
+// Copies one M-code row, 8 bytes at a time when both rows allow it.
+__device__ inline void copy_codes(int16_t* __restrict__ dst, const int16_t* __restrict__ src,
+                                  int M) {
+  if ((M & 3) == 0) {
+    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+    for (int i = 0; i < (M >> 2); ++i) d[i] = s[i];
+  } else if ((M & 1) == 0) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (int i = 0; i < (M >> 1); ++i) d[i] = s[i];
+  } else {
+    for (int m = 0; m < M; ++m) dst[m] = src[m];
+  }
+}
+
+// Bucket records from heads: key and start offset into the sorted ids, and
 // codes (row-major [B][M]).
 __global__ __launch_bounds__(256) void k_bucket_fill(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl, long long n_all,
-    long long N, int M, int tb, const int16_t* __restrict__ codes, uint64_t* __restrict__ ukeys,
-    uint32_t* __restrict__ bstart, int16_t* __restrict__ bcodes) {
+    long long N, int M, int tb, const int16_t* __restrict__ codes, BucketRec* __restrict__ recs,
+    int16_t* __restrict__ bcodes) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n_all || !flag[i]) return;
   const uint32_t k = excl[i];
-  ukeys[k] = keys[i];
-  bstart[k] = (uint32_t)i;
+  recs[k].key = keys[i];
+  recs[k].start = (uint32_t)i;
   const int t = (int)(keys[i] >> (64 - tb));
-  const int16_t* src = codes + ((long long)t * N + ids[i]) * M;
-  int16_t* dst = bcodes + (long long)k * M;
-  for (int m = 0; m < M; ++m) dst[m] = src[m];
+  copy_codes(bcodes + (long long)k * M, codes + ((long long)t * N + ids[i]) * M, M);
+}
+
+// Bucket sizes from consecutive starts.
+__global__ __launch_bounds__(256) void k_rec_sizes(BucketRec* __restrict__ recs, long long B,
+                                                   long long n_all) {
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (k >= B) return;
+  const uint32_t end = k + 1 < B ? recs[k + 1].start : (uint32_t)n_all;
+  recs[k].size = end - recs[k].start;
 }
 
 // First bucket of each table (positions t*N in the sorted order) and the
@@ -269,27 +292,35 @@ __global__ void k_table_bases(const uint32_t* __restrict__ flag, const uint32_t*
   }
 }
 
-// dir[v] = first bucket whose key >> shift >= v, for v in [0, nslots];
-// dir[nslots] = B.
-__global__ __launch_bounds__(256) void k_directory(const uint64_t* __restrict__ ukeys, long long B,
-                                                   int shift, long long nslots,
-                                                   uint32_t* __restrict__ dir) {
-  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (v > nslots) return;
-  if (v == nslots) {
-    dir[v] = (uint32_t)B;
-    return;
-  }
-  const uint64_t target = (uint64_t)v << shift;
+__device__ inline long long lower_bound_rec(const BucketRec* __restrict__ recs, long long B,
+                                            uint64_t target) {
   long long lo = 0, hi = B;
   while (lo < hi) {
-    long long mid = (lo + hi) >> 1;
-    if (ukeys[mid] < target)
+    const long long mid = (lo + hi) >> 1;
+    if (recs[mid].key < target)
       lo = mid + 1;
     else
       hi = mid;
   }
-  dir[v] = (uint32_t)lo;
+  return lo;
+}
+
+// Directory entry of every slot v in [0, nslots] (see dir_entry); slot
+// nslots is the end sentinel (first = B, count 0).
+__global__ __launch_bounds__(256) void k_directory(const BucketRec* __restrict__ recs, long long B,
+                                                   int shift, long long nslots,
+                                                   uint64_t* __restrict__ dir) {
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v > nslots) return;
+  if (v == nslots) {
+    dir[v] = dir_entry((uint32_t)B, 0, 0);
+    return;
+  }
+  const long long first = lower_bound_rec(recs, B, (uint64_t)v << shift);
+  const long long end =
+      v + 1 == nslots ? B : lower_bound_rec(recs, B, (uint64_t)(v + 1) << shift);
+  const uint32_t cnt = (uint32_t)(end - first);
+  dir[v] = dir_entry((uint32_t)first, cnt ? recs[first].key : 0ull, cnt);
 }
 
 }  // namespace culsh

@@ -1,0 +1,300 @@
+// k_query_probe8: the query front of the search with one octet (8 lanes) per
+// (query, table) and 32 queries of one table per workgroup.
+//
+// Same results as k_query_probe (one wave per pair) -- FP64 projection in the
+// oracle's order, deltas, delta order, the reference's multiprobe queue,
+// probe keys, directory lookup and full-code verify (cuLSH_Querying.cu:63-225,
+// cuLSH_Kernels.cu:124-289) -- but eight pairs share a wave and the queue's
+// cross-lane steps are DPP moves inside the octet.
+#pragma once
+#include "common.hpp"
+#include "delta_order.hpp"
+
+namespace culsh {
+
+// Per-octet LDS scratch (bytes): ds[2M] f64 (deltas, then sorted deltas),
+// perm[64] i8, rank[128] u8, base[M] i32.
+__host__ __device__ inline int probe8_octet_bytes(int M) {
+  return (16 * M + 64 + 128 + 4 * M + 15) & ~15;
+}
+// Dynamic LDS of one workgroup: 32 query vectors + 32 octet scratches.
+__host__ __device__ inline int probe8_lds_bytes(int D, int M) {
+  return ((32 * D * 4 + 15) & ~15) + 32 * probe8_octet_bytes(M);
+}
+
+// Does the probe code of `mask` (base code plus the mask's +-1 moves) equal
+// the bucket code bc?  Compares the dimensions where the bucket differs from
+// the base code with the dimensions the mask moves.
+__device__ inline bool probe_matches(const int16_t* __restrict__ bc, const int* base,
+                                     const signed char* perm, const unsigned char* rank, int M,
+                                     uint64_t mask) {
+  int nz = 0;
+  for (int m = 0; m < M; ++m) nz += (int)bc[m] != base[m];
+  int matched = 0;
+  uint64_t mm = mask;
+  while (mm) {
+    const int i = __ffsll((unsigned long long)mm) - 1;
+    mm &= mm - 1;
+    const int di = perm[i];
+    const int dim = di < M ? di : di - M;
+    const int pr = rank[di < M ? di + M : di - M];
+    if (pr != 255 && ((mask >> pr) & 1ull)) continue;  // -1 and +1 cancel
+    if ((int)bc[dim] - base[dim] != (di >= M ? 1 : -1)) return false;
+    ++matched;
+  }
+  return matched == nz;
+}
+
+// Key sum of probe `mask`: the base key sum with the moved dimensions'
+// terms replaced.
+__device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int* base,
+                                      const signed char* perm, const unsigned char* rank, int M,
+                                      uint64_t mask) {
+  uint64_t hs = basesum;
+  uint64_t mm = mask;
+  while (mm) {
+    const int i = __ffsll((unsigned long long)mm) - 1;
+    mm &= mm - 1;
+    const int di = perm[i];
+    const int dim = di < M ? di : di - M;
+    const int pr = rank[di < M ? di + M : di - M];
+    if (pr != 255 && ((mask >> pr) & 1ull)) continue;
+    const int b0 = base[dim];
+    hs += code_term(seed, dim, b0 + (di >= M ? 1 : -1)) - code_term(seed, dim, b0);
+  }
+  return hs;
+}
+
+// J = ceil(T / 8) queue slots per lane (slot s: lane s%8, register s/8);
+// MI >= ceil(M / 8) dimensions per lane (lane l owns l + 8i), a compile-time
+// bound so that the projection loop is branch-free.  Ad is the [D][Jp] f32
+// projection matrix (column t*M + m) with at least 8*MI floats of padding
+// after its last row, read through L1 (lanes past M read padding or the next
+// table's columns and discard them); the FP64 product of f32 operands is
+// exact, so widening on read is the oracle's arithmetic.  Needs
+// probe8_lds_bytes(D, M) bytes of dynamic LDS.
+template <int J, int MI>
+__global__ __launch_bounds__(256) void k_query_probe8(
+    const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
+    const float* __restrict__ Ad, int Jp, const double* __restrict__ b64, double W,
+    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
+    const BucketRec* __restrict__ recs, const int16_t* __restrict__ bcodes,
+    int* __restrict__ qb, unsigned long long* __restrict__ cnu_qt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane8 = tid & 7, oct = tid >> 3;
+  const int t = blockIdx.x % L;
+  const int q0 = (blockIdx.x / L) * 32;
+  const int q = q0 + oct;
+  const bool active = q < Q;
+  float* Ys = reinterpret_cast<float*>(smem);
+  unsigned char* orb = smem + ((32 * D * 4 + 15) & ~15) + oct * probe8_octet_bytes(M);
+  double* ds = reinterpret_cast<double*>(orb);
+  signed char* perm = reinterpret_cast<signed char*>(ds + 2 * M);
+  unsigned char* rank = reinterpret_cast<unsigned char*>(perm + 64);
+  int* base = reinterpret_cast<int*>(rank + 128);
+
+  // Stage the block's query vectors (coalesced).
+  const int nq = min(32, Q - q0);
+  for (int e = tid; e < nq * D; e += 256) Ys[e] = Qm[(long long)q0 * D + e];
+  for (int r = lane8; r < 128; r += 8) rank[r] = 255;
+  __syncthreads();
+
+  // FP64 projection, the oracle's order; lane l owns dimensions l + 8i.
+  bool valid[MI];
+  double acc[MI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i) {
+    valid[i] = lane8 + 8 * i < M;
+    acc[i] = 0.0;
+  }
+  const float* y = Ys + (active ? oct : 0) * D;
+  const float* acol = Ad + t * M + lane8;
+#pragma unroll 8
+  for (int d = 0; d < D; ++d) {
+    const double yd = (double)y[d];
+    const float* arow = acol + (long long)d * Jp;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) acc[i] = fma(yd, (double)arow[8 * i], acc[i]);
+  }
+  const uint64_t seed = seeds[t];
+  double v[MI];
+  uint64_t term = 0;
+#pragma unroll
+  for (int i = 0; i < MI; ++i) {
+    const int m = lane8 + 8 * i;
+    v[i] = valid[i] ? (acc[i] - b64[t * M + m]) / W : 0.0;
+    const int c = (int)floor(v[i]);
+    if (valid[i]) {
+      base[m] = c;
+      term += code_term(seed, m, c);
+    }
+  }
+  const uint64_t basesum = octet_sum_u64(term);
+  const int maxT = (2 * M < 64) ? 2 * M : 64;
+
+  if (T > 1) {
+    // Deltas (cuLSH_Kernels.cu:280-287): index m = "-1 on m", M+m = "+1 on m".
+    double dn[MI], up[MI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int m = lane8 + 8 * i;
+      dn[i] = v[i] - floor(v[i]);
+      up[i] = 1.0 - dn[i];
+      if (valid[i]) {
+        ds[m] = dn[i];
+        ds[M + m] = up[i];
+      }
+    }
+    wave_lds_sync();
+    // Rank of each delta = number of smaller deltas; the order is unique when
+    // no two deltas are equal.
+    int rdn[MI], rup[MI];
+    bool tie = false;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) rdn[i] = rup[i] = 0;
+#pragma unroll 4
+    for (int j = 0; j < 2 * M; ++j) {
+      const double dj = ds[j];
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int m = lane8 + 8 * i;
+        rdn[i] += dj < dn[i];
+        rup[i] += dj < up[i];
+        tie |= valid[i] && ((dj == dn[i] && j != m) || (dj == up[i] && j != M + m));
+      }
+    }
+    const bool otie = octet_min_u32(tie ? 0u : 1u) == 0u;
+    wave_lds_sync();  // every lane has read ds before it is overwritten
+    if (!otie) {
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int m = lane8 + 8 * i;
+        if (valid[i]) {
+          if (rdn[i] < maxT) {
+            perm[rdn[i]] = (signed char)m;
+            ds[rdn[i]] = dn[i];
+            rank[m] = (unsigned char)rdn[i];
+          }
+          if (rup[i] < maxT) {
+            perm[rup[i]] = (signed char)(M + m);
+            ds[rup[i]] = up[i];
+            rank[M + m] = (unsigned char)rup[i];
+          }
+        }
+      }
+    } else {
+      // Exact ties: the reference heap's order (one lane), then the sorted
+      // deltas from the lanes' registers.
+      if (lane8 == 0) heap_order_deltas(ds, 2 * M, maxT, perm);
+      wave_lds_sync();
+      for (int r = lane8; r < maxT; r += 8) rank[perm[r]] = (unsigned char)r;
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int m = lane8 + 8 * i;
+        if (valid[i]) {
+          if (rank[m] != 255) ds[rank[m]] = dn[i];
+          if (rank[M + m] != 255) ds[rank[M + m]] = up[i];
+        }
+      }
+    }
+  }
+  wave_lds_sync();
+
+  // Multiprobe queue (cuLSH_Kernels.cu:216-255).  Scores are non-negative
+  // doubles, kept as their bit patterns (same order); empty slots hold ~0.
+  uint64_t qs[J], qc[J], pm[J];
+#pragma unroll
+  for (int r = 0; r < J; ++r) {
+    qs[r] = ~0ull;
+    qc[r] = 0ull;
+    pm[r] = 0ull;
+  }
+  if (T > 1) {
+    if (lane8 == 0) {
+      qc[0] = 1ull;
+      qs[0] = (uint64_t)__double_as_longlong(ds[0]);
+    }
+    int size = 1;
+    for (int p = 1; p < T; ++p) {
+      uint64_t best = ~0ull;
+      int br = 0;
+#pragma unroll
+      for (int r = 0; r < J; ++r)
+        if (qs[r] < best) {
+          best = qs[r];
+          br = r;
+        }
+      const uint64_t omin = octet_min_u64(best);
+      const uint32_t myslot = best == omin ? (uint32_t)(br * 8 + lane8) : 0xFFFFFFFFu;
+      const uint32_t ws = octet_min_u32(myslot);
+      const int wl = (int)(ws & 7), wr = (int)(ws >> 3);
+      uint64_t sel = 0ull;
+#pragma unroll
+      for (int r = 0; r < J; ++r)
+        if (r == wr) sel = qc[r];
+      const uint64_t probe = octet_or_u64(lane8 == wl ? sel : 0ull);
+      if (lane8 == (p & 7)) {
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+          if (r == (p >> 3)) pm[r] = probe;
+      }
+      const int maxPos = 63 - __clzll((long long)probe);
+      if (maxPos < maxT - 1) {
+        const double e = __longlong_as_double((long long)omin) + ds[maxPos + 1];
+        const uint64_t pe = probe | (1ull << (maxPos + 1));
+        const double sh = e - ds[maxPos];
+        const uint64_t ps = pe - (1ull << maxPos);
+        if (lane8 == wl) {
+#pragma unroll
+          for (int r = 0; r < J; ++r)
+            if (r == wr) {
+              qc[r] = pe;
+              qs[r] = (uint64_t)__double_as_longlong(e);
+            }
+        }
+        if (lane8 == (size & 7)) {
+#pragma unroll
+          for (int r = 0; r < J; ++r)
+            if (r == (size >> 3)) {
+              qc[r] = ps;
+              qs[r] = (uint64_t)__double_as_longlong(sh);
+            }
+        }
+        ++size;
+      }
+    }
+  }
+
+  // Keys, lookup, verify: lane l handles probes l + 8r.  All of a lane's
+  // directory loads are issued before any is resolved.
+  unsigned long long sizesum = 0;
+  const long long qbBase = (long long)t * Q * T + (long long)q * T;
+  uint64_t key[J], ent[J];
+#pragma unroll
+  for (int r = 0; r < J; ++r) {
+    const int p = r * 8 + lane8;
+    key[r] = (active && p < T)
+                 ? make_key(probe_hash(basesum, seed, base, perm, rank, M, pm[r]), t, tb)
+                 : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < J; ++r) {
+    const int p = r * 8 + lane8;
+    ent[r] = (active && p < T) ? dir[key[r] >> shift] : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < J; ++r) {
+    const int p = r * 8 + lane8;
+    if (active && p < T) {
+      long long k = lookup_resolve(key[r], ent[r], dir, shift, recs);
+      if (k >= 0 && !probe_matches(bcodes + k * M, base, perm, rank, M, pm[r])) k = -1;
+      qb[qbBase + p] = (int)k;
+      if (k >= 0) sizesum += recs[k].size;
+    }
+  }
+  sizesum = octet_sum_u64(sizesum);
+  if (active && lane8 == 0) cnu_qt[(long long)q * L + t] = sizesum;
+}
+
+}  // namespace culsh

@@ -15,55 +15,10 @@
 //                Kernels.cu:301-348 (distances), Kernels.cu:356-456 (top-K).
 #pragma once
 #include "common.hpp"
+#include "delta_order.hpp"
+#include "probe8.hpp"
 
 namespace culsh {
-
-__device__ inline int ceil_log2_i(int x) {
-  int l = 0;
-  while ((1 << l) < x) ++l;
-  return l;
-}
-
-// Sift-down of the bounded max-heap of delta indices (-1 = empty = +inf),
-// cuLSH_Kernels.cu:144-175.
-__device__ inline void sift_deltas(int* perm, const double* d, int total, int levels) {
-  int parent = 0;
-  for (int level = 0; level < levels - 1; ++level) {
-    int left = parent * 2 + 1, right = left + 1, mx = parent;
-    if (left < total && perm[left] == -1)
-      mx = left;
-    else if (right < total && perm[right] == -1)
-      mx = right;
-    else {
-      if (left < total && d[perm[left]] > d[perm[mx]]) mx = left;
-      if (right < total && d[perm[right]] > d[perm[mx]]) mx = right;
-    }
-    if (mx == parent) break;
-    int tmp = perm[parent];
-    perm[parent] = perm[mx];
-    perm[mx] = tmp;
-    parent = mx;
-  }
-}
-
-// The reference's delta ordering (heap select of the maxT smallest, then
-// heapsort, cuLSH_Kernels.cu:202-214), run by one lane; used when deltas tie
-// exactly, where the heap's mechanics decide the order.
-__device__ inline void heap_order_deltas(const double* d, int twoM, int maxT, int* perm) {
-  const int levels = ceil_log2_i(twoM + 1);
-  for (int i = 0; i < maxT; ++i) perm[i] = -1;
-  for (int i = 0; i < twoM; ++i)
-    if (perm[0] == -1 || d[i] < d[perm[0]]) {
-      perm[0] = i;
-      sift_deltas(perm, d, maxT, levels);
-    }
-  for (int i = maxT - 1; i; --i) {
-    int tmp = perm[0];
-    perm[0] = perm[i];
-    perm[i] = tmp;
-    sift_deltas(perm, d, i, levels);
-  }
-}
 
 // One wave per (query, table).  J = ceil(T / 64) queue slots per lane.
 // qb[t][q*T + p] receives the global bucket index of probe p (or -1);
@@ -72,8 +27,8 @@ template <int J>
 __global__ __launch_bounds__(256) void k_query_probe(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
     const double* __restrict__ At64, const double* __restrict__ b64, double W,
-    const uint64_t* __restrict__ seeds, int tb, int shift, const uint32_t* __restrict__ dir,
-    const uint64_t* __restrict__ ukeys, const uint32_t* __restrict__ bstart,
+    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
+    const BucketRec* __restrict__ recs,
     const int16_t* __restrict__ bcodes, int* __restrict__ qb,
     unsigned long long* __restrict__ cnu_qt) {
   __shared__ double s_ds[4][64];
@@ -253,7 +208,7 @@ __global__ __launch_bounds__(256) void k_query_probe(
         const int b0 = sbase[dim];
         hs += code_term(seed, dim, b0 + (di >= M ? 1 : -1)) - code_term(seed, dim, b0);
       }
-      long long k = lookup_key(make_key(hs, t, tb), dir, shift, ukeys);
+      long long k = lookup_key(make_key(hs, t, tb), dir, shift, recs);
       if (k >= 0) {
         const int16_t* bc = bcodes + k * M;
         for (int m = 0; m < M; ++m) {
@@ -268,7 +223,7 @@ __global__ __launch_bounds__(256) void k_query_probe(
         }
       }
       qb[qbBase + p] = (int)k;
-      if (k >= 0) sizesum += bstart[k + 1] - bstart[k];
+      if (k >= 0) sizesum += recs[k].size;
     }
   }
   sizesum = wave_sum_u64(sizesum);
@@ -296,7 +251,7 @@ __device__ inline double dist64(const float* __restrict__ q, const float* __rest
 // for the caller.
 __global__ __launch_bounds__(256) void k_candidates(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
-    const int* __restrict__ qb, const uint32_t* __restrict__ bstart,
+    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
     const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, int* __restrict__ overflow,
@@ -336,8 +291,8 @@ __global__ __launch_bounds__(256) void k_candidates(
         const int t = e / T, p = e - t * T;
         const int k = qb[(long long)t * Q * T + (long long)q * T + p];
         if (k >= 0) {
-          st = bstart[k];
-          sz = bstart[k + 1] - st;
+          st = recs[k].start;
+          sz = recs[k].size;
         }
       }
       uint32_t inc = sz;

@@ -60,6 +60,24 @@ def test_wide_probe_queue_and_many_dims():
         assert_search_equal(g, o)
 
 
+def test_wave_kernel_paths():
+    """T > 128 and D*M beyond the octet kernel's LDS staging use the
+    wave-per-(query, table) probe kernel."""
+    print("wave kernel paths", flush=True)
+    X = wl.mixture(3000, 24, 10, 0.3, 71)
+    Y = wl.queries_near(X, 30, 0.05, 72)
+    A, b = wl.projection(2, 24, 6, 1.0, 73)
+    g = run_gpu(X, Y, A, b, 2, 6, 1.0, 5, 150)
+    o = run_oracle(X, Y, A, b, 2, 6, 1.0, 5, 150)
+    assert_search_equal(g, o)
+    X = wl.mixture(2000, 520, 10, 0.05, 74)
+    Y = wl.queries_near(X, 30, 0.01, 75)
+    A, b = wl.projection(2, 520, 40, 4.0, 76)
+    g = run_gpu(X, Y, A, b, 2, 40, 4.0, 5, 20)
+    o = run_oracle(X, Y, A, b, 2, 40, 4.0, 5, 20)
+    assert_search_equal(g, o)
+
+
 def test_exact_delta_ties():
     """Duplicated projection columns give exactly tied deltas: the delta order
     then follows the reference heap's mechanics."""
