@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "delta_order.hpp"
 #include "probe8.hpp"
+#include "sort.hpp"
 
 namespace culsh {
 
@@ -233,16 +234,95 @@ __global__ __launch_bounds__(256) void k_query_probe(
 constexpr int kHcap = 8192;  // LDS hash slots per query
 constexpr int kUcap = 4096;  // unique candidates per query (LDS path)
 constexpr int kScap = 1024;  // candidates re-evaluated in FP64 per query
+constexpr int kYcap = 1024;  // query dimensions staged in LDS
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 // FP64 squared L2 in the oracle's order (sequential FMA over d ascending).
+// Rows are fetched 32 floats at a time (8 independent 16-byte loads) ahead
+// of the dependent FMA chain when D is a multiple of 32.
 __device__ inline double dist64(const float* __restrict__ q, const float* __restrict__ x, int D) {
   double acc = 0.0;
+  if ((D & 31) == 0) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    for (int d0 = 0; d0 < D; d0 += 32) {
+      float4 xv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) xv[k] = x4[(d0 >> 2) + k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float* xs = reinterpret_cast<const float*>(&xv[k]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double df = (double)q[d0 + 4 * k + e] - (double)xs[e];
+          acc = fma(df, df, acc);
+        }
+      }
+    }
+    return acc;
+  }
+#pragma unroll 8
   for (int d = 0; d < D; ++d) {
     const double df = (double)q[d] - (double)x[d];
     acc = fma(df, df, acc);
   }
   return acc;
+}
+
+// Direct FP32 squared L2 of the U candidate rows ids[0..U) against q
+// (cuLSH_Kernels.cu:319-346): 16 lanes per row, four rows per 16-lane group
+// with their loads issued together.  Writes d32[u].
+__device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __restrict__ q,
+                                    int D, const uint32_t* ids, int U, float* d32) {
+  const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
+  if ((D & 3) == 0) {
+    const int D4 = D >> 2;
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    for (int u0 = g * 4; u0 < U; u0 += 64) {
+      const float4* xr[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        xr[c] = reinterpret_cast<const float4*>(X + (long long)ids[min(u0 + c, U - 1)] * D);
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+      for (int f = lig; f < D4; f += 16) {
+        float4 a[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c] = xr[c][f];
+        const float4 b = q4[f];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float d0 = b.x - a[c].x, d1 = b.y - a[c].y, d2 = b.z - a[c].z,
+                      d3 = b.w - a[c].w;
+          acc[c] = fmaf(d0, d0, acc[c]);
+          acc[c] = fmaf(d1, d1, acc[c]);
+          acc[c] = fmaf(d2, d2, acc[c]);
+          acc[c] = fmaf(d3, d3, acc[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 16);
+      }
+      if (lig == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (u0 + c < U) d32[u0 + c] = acc[c];
+      }
+    }
+  } else {
+    for (int u = g; u < U; u += 16) {
+      const float* x = X + (long long)ids[u] * D;
+      float acc = 0.f;
+      for (int d = lig; d < D; d += 16) {
+        const float df = q[d] - x[d];
+        acc = fmaf(df, df, acc);
+      }
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+      if (lig == 0) d32[u] = acc;
+    }
+  }
 }
 
 // One workgroup (256 threads) per query.  rel_bound bounds the relative
@@ -264,6 +344,8 @@ __global__ __launch_bounds__(256) void k_candidates(
   float* s_d32 = reinterpret_cast<float*>(s_hash);
   double* s_sd = reinterpret_cast<double*>(s_hash + kUcap);            // kScap doubles
   uint32_t* s_sid = s_hash + kUcap + 2 * kScap;                         // kScap ids
+  float* s_y = reinterpret_cast<float*>(s_hash + kUcap + 3 * kScap);     // kYcap floats
+  static_assert(kUcap + 3 * kScap + kYcap <= kHcap, "post-insertion LDS layout");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int q = blockIdx.x;
   const float* y = Qm + (long long)q * D;
@@ -345,35 +427,15 @@ __global__ __launch_bounds__(256) void k_candidates(
   const int U = (int)s_count;
   if (tid == 0) cu_q[q] = (unsigned long long)U;
 
-  // Direct FP32 squared L2 (cuLSH_Kernels.cu:319-346), 16 lanes per candidate.
-  {
-    const int g = tid >> 4, lig = tid & 15;
-    const bool vec = (D & 3) == 0;
-    for (int u = g; u < U; u += 16) {
-      const float* x = X + (long long)s_u[u] * D;
-      float acc = 0.f;
-      if (vec) {
-        const float4* x4 = reinterpret_cast<const float4*>(x);
-        const float4* y4 = reinterpret_cast<const float4*>(y);
-        for (int f = lig; f < (D >> 2); f += 16) {
-          const float4 a = x4[f], bq = y4[f];
-          float d0 = bq.x - a.x, d1 = bq.y - a.y, d2 = bq.z - a.z, d3 = bq.w - a.w;
-          acc = fmaf(d0, d0, acc);
-          acc = fmaf(d1, d1, acc);
-          acc = fmaf(d2, d2, acc);
-          acc = fmaf(d3, d3, acc);
-        }
-      } else {
-        for (int d = lig; d < D; d += 16) {
-          const float df = y[d] - x[d];
-          acc = fmaf(df, df, acc);
-        }
-      }
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
-      if (lig == 0) s_d32[u] = acc;
-    }
-  }
+  // The query vector moves to LDS (the hash slots are free now).
+  const bool y_in_lds = D <= kYcap;
+  if (y_in_lds)
+    for (int d = tid; d < D; d += 256) s_y[d] = y[d];
+  __syncthreads();
+  if (y_in_lds)
+    l2_fp32_rows(X, s_y, D, s_u, U, s_d32);
+  else
+    l2_fp32_rows(X, y, D, s_u, U, s_d32);
   __syncthreads();
 
   // K-th smallest FP32 distance by radix select over the float bits
@@ -394,15 +456,15 @@ __global__ __launch_bounds__(256) void k_candidates(
         if ((bits & pmask) == prefix) atomicAdd(&s_hist[(bits >> shiftb) & 255u], 1u);
       }
       __syncthreads();
-      if (tid == 0) {
-        unsigned rem = s_rem, cum = 0;
-        int dg = 0;
-        for (; dg < 256; ++dg) {
-          if (cum + s_hist[dg] >= rem) break;
-          cum += s_hist[dg];
-        }
-        s_rem = rem - cum;
-        s_prefix = prefix | ((uint32_t)dg << shiftb);
+      // The digit whose bin holds the rem-th smallest: bins are scanned in
+      // parallel, exactly one thread finds rem inside its bin.
+      const unsigned rem = s_rem;
+      const uint32_t h = s_hist[tid];
+      uint32_t tot;
+      const uint32_t before = block_exclusive_scan_256(h, s_misc, tot);
+      if (h > 0 && before < rem && rem <= before + h) {
+        s_rem = rem - before;
+        s_prefix = prefix | ((uint32_t)tid << shiftb);
       }
       pmask |= 255u << shiftb;
       __syncthreads();
@@ -418,7 +480,8 @@ __global__ __launch_bounds__(256) void k_candidates(
       const unsigned i = atomicAdd(&s_scount, 1u);
       if (i < (unsigned)kScap) {
         const uint32_t id = s_u[u];
-        s_sd[i] = dist64(y, X + (long long)id * D, D);
+        s_sd[i] = y_in_lds ? dist64(s_y, X + (long long)id * D, D)
+                           : dist64(y, X + (long long)id * D, D);
         s_sid[i] = id;
       } else {
         s_ovf = 1;
