@@ -49,12 +49,21 @@ def build_culsh():
     os.replace(tmp, out)
 
 
+def build_cli():
+    """culsh_cli: the reference CLI's contract, a plain C++ caller of cuLSH.h."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "cli.cpp"), "-L", LIBDIR, "-lculsh",
+          f"-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIBDIR, "culsh_cli")])
+
+
 def build_all():
     build_workload()
     build_culsh()
+    build_cli()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "culsh": build_culsh, "workload": build_workload}[w]()
+        {"all": build_all, "culsh": build_culsh, "workload": build_workload, "cli": build_cli}[w]()

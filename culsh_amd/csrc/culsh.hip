@@ -630,16 +630,211 @@ int culsh_ht_set_form(culsh_ht* ht, int form) {
   return CULSH_OK;
 }
 
+// File layout of HashTables::save / load (cuLSH_HashTables.cu:395-483),
+// native endianness, no header:
+//   int32 N, D, L, M; float W
+//   per table: float A_t[D*M], float b_t[M]
+//   uint32 B[L]
+//   per table: uint32 totalIndices[N], startingIndices[B_t], sizes[B_t],
+//              float codes[B_t*M] (column-major [B_t x M])
 int culsh_ht_save(culsh_ht* ht, const char* filename, FILE* dbg) {
-  (void)dbg;
   if (!ht || !filename) return fail(CULSH_E_ARG, "null argument");
-  return fail(CULSH_E_LIMIT, "save is not available in this build");
+  if (dbg)
+    fprintf(dbg, "[HashTables::save]\tSaving indexing structure to file \"%s\"...\n", filename);
+  if (!ht->has_params)
+    return fail(CULSH_E_STATE,
+                "Pointers to matrices [A] and/or vectors [b] are null. Have you reset the index "
+                "structure properly???");
+  if (!ht->indexed)
+    return fail(CULSH_E_STATE, "Pointers to bucket objects are null. Have you performed indexing???");
+  TRY(ensure_canonical(ht));
+  FILE* fp = fopen(filename, "wb");
+  if (!fp)
+    return fail(CULSH_E_IO, "Could not create file \"%s\" in order to store the hash tables...",
+                filename);
+  const int L = ht->L, M = ht->M, D = ht->D, N = ht->N;
+  bool ok = fwrite(&ht->N, 4, 1, fp) == 1 && fwrite(&ht->D, 4, 1, fp) == 1 &&
+            fwrite(&ht->L, 4, 1, fp) == 1 && fwrite(&ht->M, 4, 1, fp) == 1 &&
+            fwrite(&ht->W, 4, 1, fp) == 1;
+  for (int t = 0; ok && t < L; ++t)
+    ok = fwrite(ht->A.data() + (size_t)t * D * M, 4, (size_t)D * M, fp) == (size_t)D * M &&
+         fwrite(ht->b.data() + (size_t)t * M, 4, M, fp) == (size_t)M;
+  ok = ok && fwrite(ht->hB.data(), 4, L, fp) == (size_t)L;
+  for (int t = 0; ok && t < L; ++t) {
+    const size_t B = ht->hB[t];
+    ok = fwrite(ht->hTotal[t].data(), 4, N, fp) == (size_t)N &&
+         fwrite(ht->hStart[t].data(), 4, B, fp) == B &&
+         fwrite(ht->hSize[t].data(), 4, B, fp) == B &&
+         fwrite(ht->hCodes[t].data(), 4, B * M, fp) == B * M;
+  }
+  ok = ok && !ferror(fp);
+  if (fclose(fp) != 0) ok = false;
+  if (!ok) return fail(CULSH_E_IO, "writing \"%s\" failed", filename);
+  return CULSH_OK;
 }
 
+namespace {
+
+// Device index from lexicographic bucket arrays (the saved form): bucket
+// keys, records sorted by key, member ids in key order, codes, directory,
+// and the lexicographic numbering for the host mirrors.
+int index_from_buckets(culsh_ht* ht, FILE* dbg) {
+  const long long N = ht->N, L = ht->L, M = ht->M;
+  ht->tb = std::max(1, ceil_log2_host(L));
+  long long Btot = 0;
+  std::vector<long long> lexbase(L + 1, 0);
+  for (int t = 0; t < L; ++t) lexbase[t + 1] = lexbase[t] + ht->hB[t];
+  Btot = lexbase[L];
+  std::vector<int16_t> codes16((size_t)Btot * M);
+  for (int t = 0; t < L; ++t) {
+    const long long Bt = ht->hB[t];
+    for (long long k = 0; k < Bt; ++k)
+      for (int m = 0; m < M; ++m) {
+        const float c = ht->hCodes[t][(size_t)m * Bt + k];
+        if (!(c == std::floor(c)) || std::fabs(c) > (float)(kCodeMax - 1))
+          return fail(CULSH_E_RANGE, "table %d bucket %lld: code %g is not a 16-bit integer", t,
+                      k, (double)c);
+        codes16[(size_t)(lexbase[t] + k) * M + m] = (int16_t)c;
+      }
+  }
+  std::vector<BucketRec> recs(Btot);
+  std::vector<uint32_t> sids((size_t)L * N);
+  std::vector<int> renum(Btot);
+  std::vector<int16_t> bcodes((size_t)Btot * M);
+  for (int attempt = 0;; ++attempt) {
+    if (attempt > 4) return fail(CULSH_E_DEVICE, "bucket key collisions persist after reseeding");
+    std::vector<uint64_t> seeds(L);
+    for (int t = 0; t < L; ++t) seeds[t] = mix64(ht->hash_seed + (uint64_t)t);
+    std::vector<std::pair<uint64_t, uint32_t>> order(Btot);  // (key, lexicographic global index)
+    for (int t = 0; t < L; ++t)
+      for (long long k = lexbase[t]; k < lexbase[t + 1]; ++k) {
+        uint64_t h = 0;
+        for (int m = 0; m < M; ++m) h += code_term(seeds[t], m, codes16[(size_t)k * M + m]);
+        order[k] = {make_key(h, t, ht->tb), (uint32_t)k};
+      }
+    std::sort(order.begin(), order.end());
+    bool collision = false;
+    for (long long i = 1; i < Btot; ++i)
+      if (order[i].first == order[i - 1].first) collision = true;
+    if (collision) {
+      if (dbg) fprintf(dbg, "[HashTables::load]\tbucket key collision, reseeding\n");
+      ht->hash_seed = mix64(ht->hash_seed + 0x9E3779B97F4A7C15ull);
+      continue;
+    }
+    uint32_t pos = 0;
+    for (long long g = 0; g < Btot; ++g) {
+      const uint32_t lk = order[g].second;
+      const int t = (int)(order[g].first >> (64 - ht->tb));
+      const long long k = lk - lexbase[t];
+      const uint32_t st = ht->hStart[t][k], sz = ht->hSize[t][k];
+      recs[g] = BucketRec{order[g].first, pos, sz};
+      for (uint32_t r = 0; r < sz; ++r) sids[pos + r] = ht->hTotal[t][st + r];
+      pos += sz;
+      std::memcpy(&bcodes[(size_t)g * M], &codes16[(size_t)lk * M], M * sizeof(int16_t));
+      renum[g] = (int)k;
+    }
+    HIP_TRY(hipMemcpy(ht->dseeds.p, seeds.data(), L * 8, hipMemcpyHostToDevice));
+    break;
+  }
+  ht->table_base.assign(L + 1, 0);
+  for (int t = 0; t <= L; ++t) ht->table_base[t] = (uint32_t)lexbase[t];
+  TRY(ht->recs.exact(Btot));
+  TRY(ht->bcodes.exact(Btot * M));
+  TRY(ht->sids.exact((size_t)L * N));
+  TRY(ht->renumber.exact(std::max<long long>(Btot, 1)));
+  HIP_TRY(hipMemcpy(ht->recs.p, recs.data(), Btot * sizeof(BucketRec), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->bcodes.p, bcodes.data(), bcodes.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->sids.p, sids.data(), sids.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->renumber.p, renum.data(), Btot * 4, hipMemcpyHostToDevice));
+  TRY(build_directory(ht));
+  HIP_TRY(hipStreamSynchronize(ht->stream));
+  HIP_TRY(hipGetLastError());
+  ht->indexed = true;
+  ht->canon = true;
+  return CULSH_OK;
+}
+
+int load_file(culsh_ht* ht, FILE* fp, const char* filename, FILE* dbg) {
+  int N = 0, D = 0, L = 0, M = 0;
+  float W = 0.f;
+  auto rd = [fp](void* p, size_t sz, size_t n) { return fread(p, sz, n, fp) == n; };
+  if (!rd(&N, 4, 1) || !rd(&D, 4, 1) || !rd(&L, 4, 1) || !rd(&M, 4, 1) || !rd(&W, 4, 1))
+    return fail(CULSH_E_IO, "\"%s\": truncated header", filename);
+  if (dbg)
+    fprintf(dbg,
+            "[HashTables::load]\tIndexing structure loaded from file \"%s\", (N, D, L, M, W) = "
+            "(%d, %d, %d, %d, %.2f)\n",
+            filename, N, D, L, M, W);
+  TRY(set_params(ht, N, D, L, M, W));
+  ht->A.resize((size_t)L * D * M);
+  ht->b.resize((size_t)L * M);
+  for (int t = 0; t < L; ++t)
+    if (!rd(ht->A.data() + (size_t)t * D * M, 4, (size_t)D * M) ||
+        !rd(ht->b.data() + (size_t)t * M, 4, M))
+      return fail(CULSH_E_IO, "\"%s\": truncated projection of table %d", filename, t);
+  TRY(upload_params(ht));
+  ht->has_params = true;
+  ht->hB.assign(L, 0);
+  if (!rd(ht->hB.data(), 4, L)) return fail(CULSH_E_IO, "\"%s\": truncated bucket counts", filename);
+  ht->hTotal.assign(L, {});
+  ht->hStart.assign(L, {});
+  ht->hSize.assign(L, {});
+  ht->hCodes.assign(L, {});
+  for (int t = 0; t < L; ++t) {
+    const size_t B = ht->hB[t];
+    if (B == 0 || B > (size_t)N)
+      return fail(CULSH_E_IO, "\"%s\": table %d has %zu buckets for N = %d", filename, t, B, N);
+    ht->hTotal[t].resize(N);
+    ht->hStart[t].resize(B);
+    ht->hSize[t].resize(B);
+    ht->hCodes[t].resize(B * M);
+    if (!rd(ht->hTotal[t].data(), 4, N) || !rd(ht->hStart[t].data(), 4, B) ||
+        !rd(ht->hSize[t].data(), 4, B) || !rd(ht->hCodes[t].data(), 4, B * M))
+      return fail(CULSH_E_IO, "\"%s\": truncated buckets of table %d", filename, t);
+    // Consistency: sizes tile [0, N) in order and every id appears once.
+    uint64_t pos = 0;
+    for (size_t k = 0; k < B; ++k) {
+      if (ht->hStart[t][k] != pos || ht->hSize[t][k] == 0)
+        return fail(CULSH_E_IO, "\"%s\": table %d bucket %zu has an inconsistent start/size",
+                    filename, t, k);
+      pos += ht->hSize[t][k];
+    }
+    if (pos != (uint64_t)N)
+      return fail(CULSH_E_IO, "\"%s\": table %d bucket sizes sum to %llu, not N", filename, t,
+                  (unsigned long long)pos);
+    std::vector<char> seen(N, 0);
+    for (int i = 0; i < N; ++i) {
+      const uint32_t id = ht->hTotal[t][i];
+      if (id >= (uint32_t)N || seen[id])
+        return fail(CULSH_E_IO, "\"%s\": table %d lists id %u twice or out of range", filename, t,
+                    id);
+      seen[id] = 1;
+    }
+  }
+  return index_from_buckets(ht, dbg);
+}
+
+}  // namespace
+
 int culsh_ht_load(culsh_ht* ht, const char* filename, FILE* dbg) {
-  (void)dbg;
   if (!ht || !filename) return fail(CULSH_E_ARG, "null argument");
-  return fail(CULSH_E_LIMIT, "load is not available in this build");
+  if (dbg)
+    fprintf(dbg, "[HashTables::reset]\tLoading indexing structure from file \"%s\"...\n",
+            filename);
+  FILE* fp = fopen(filename, "rb");
+  if (!fp)
+    return fail(CULSH_E_IO,
+                "Could not open file \"%s\" in order to load the hash tables, or file does not "
+                "exist...",
+                filename);
+  int rc = load_file(ht, fp, filename, dbg);
+  fclose(fp);
+  if (rc != CULSH_OK) {
+    std::string keep = g_err;
+    ht->drop_index();
+    g_err = keep;
+  }
+  return rc;
 }
 
 }  // extern "C"
@@ -655,7 +850,8 @@ struct culsh_st {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   DevBuf<float> dQ, dX_alt, ddists;
   DevBuf<int> dqb, dids, dov, dqb_local;
-  DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt;
+  DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt, dmax;
+  DevBuf<uint32_t> dscratch;  // working sets of overflowed queries
   DevBuf<int> dnov;
   std::vector<int> ids;
   std::vector<float> dists;
@@ -688,8 +884,49 @@ int st_setup(culsh_st* st) {
   return CULSH_OK;
 }
 
-// The device search pipeline on `s`: probe kernel, candidate kernel.
-// Synchronises once to check the per-query capacity flags.
+// Queries whose candidates overflowed the LDS kernel: global-memory working
+// sets sized from their largest C_nu, as many workgroups as fit in a 4 GiB
+// scratch budget.
+int run_overflow(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids,
+                 float* d_dists, hipStream_t s, int nov) {
+  culsh_ht* ht = st->ht;
+  const int L = ht->L, D = ht->D, T = st->T, K = st->K;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dmax.p, 1);
+  hipLaunchKernelGGL(k_max_cnu, dim3(grid_for(nov)), dim3(256), 0, s, (const int*)st->dov.p, nov,
+                     (const unsigned long long*)st->dcnu_qt.p, L, st->dmax.p);
+  unsigned long long maxcnu = 0;
+  HIP_TRY(hipMemcpyAsync(&maxcnu, st->dmax.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (maxcnu >= (1ull << 30))
+    return fail(CULSH_E_LIMIT, "a query has %llu candidates (limit 2^30)", maxcnu);
+  const int hbits = std::max(13, ceil_log2_host((long long)(2 * maxcnu)));
+  const unsigned ucap = (unsigned)std::max<unsigned long long>(maxcnu, 1);
+  const size_t region = (1ull << hbits) + 2ull * ucap;  // u32 words per workgroup
+  const size_t budget = (4ull << 30) / 4;
+  const int G = (int)std::max<size_t>(1, std::min<size_t>((size_t)nov, budget / region));
+  TRY(st->dscratch.ensure(region * G));
+  HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
+  const double rel = 2.0 * (double)(D + 8) * 0x1.0p-24;
+  hipLaunchKernelGGL(k_candidates_global, dim3(G), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
+                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
+                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                     d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p, nov,
+                     st->dscratch.p, hbits, ucap, rel, st->dcnt.p, st->dnov.p);
+  HIP_TRY(hipGetLastError());
+  int err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (err)
+    return fail(CULSH_E_LIMIT,
+                "a query has more than %d candidates within the FP32 error bound of its K-th "
+                "distance",
+                kScapG);
+  return CULSH_OK;
+}
+
+// The device search pipeline on `s`: probe kernel, candidate kernel (and the
+// global-memory kernel for queries that overflow it).  Synchronises to check
+// the per-query capacity flags.
 int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
                hipStream_t s, bool timed) {
   culsh_ht* ht = st->ht;
@@ -703,6 +940,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   TRY(st->dov.ensure(Q));
   TRY(st->dnov.ensure(1));
   TRY(st->dcnt.ensure(1));
+  TRY(st->dmax.ensure(1));
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
   if (timed) HIP_TRY(hipEventRecord(st->ev[0], s));
@@ -773,11 +1011,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   int nov = 0;
   HIP_TRY(hipMemcpyAsync(&nov, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (nov > 0)
-    return fail(CULSH_E_LIMIT,
-                "%d queries exceed the per-query candidate capacity (%d unique, %d near the K-th "
-                "distance)",
-                nov, kUcap, kScap);
+  if (nov > 0) TRY(run_overflow(st, dQ, dX, Q, d_ids, d_dists, s, nov));
   return CULSH_OK;
 }
 

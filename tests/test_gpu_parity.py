@@ -78,6 +78,21 @@ def test_wave_kernel_paths():
     assert_search_equal(g, o)
 
 
+def test_large_candidate_sets_use_global_path():
+    """Wide buckets (large W) give tens of thousands of unique candidates per
+    query, beyond the LDS working set; K=1500 exceeds the LDS recheck set.
+    Both go through the global-memory candidate kernel."""
+    print("large candidate sets", flush=True)
+    X = wl.mixture(30000, 16, 3, 0.3, 81)
+    Y = wl.queries_near(X, 40, 0.05, 82)
+    A, b = wl.projection(3, 16, 3, 8.0, 83)
+    for K, T in [(10, 4), (1500, 2)]:
+        g = run_gpu(X, Y, A, b, 3, 3, 8.0, K, T)
+        o = run_oracle(X, Y, A, b, 3, 3, 8.0, K, T)
+        assert int(o["cu"].max()) > 4096
+        assert_search_equal(g, o)
+
+
 def test_exact_delta_ties():
     """Duplicated projection columns give exactly tied deltas: the delta order
     then follows the reference heap's mechanics."""
