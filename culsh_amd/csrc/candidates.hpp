@@ -1,25 +1,29 @@
 // Candidate stage of the search: union of the matched buckets with per-query
-// dedup, direct FP32 squared L2, selection of the K-th distance, FP64
+// dedup, approximate squared L2, selection of the K-th distance, FP64
 // re-evaluation near it, and the reference's top-K tie rule.  Restates
 // cuLSH_SearchTables.cu:249-255,321-389 (candidates), cuLSH_Kernels.cu:301-348
 // (distances) and cuLSH_Kernels.cu:356-456 (top-K).
 //
-// One workgroup (256 threads) per query.  query_candidates() works on buffers
-// the caller places: k_candidates keeps everything in LDS (the common case);
-// k_candidates_global serves the queries that overflow it from a per-
-// workgroup slice of global scratch.
+// Plain form: one workgroup (256 threads) per query, direct FP32 distances
+// (k_candidates, all in LDS; k_candidates_global for queries that overflow
+// it).  Precision form: one workgroup per group of 16 locality-ordered
+// queries; the union of their candidates is read once and distances come
+// from the norm expansion on FP32 MFMA (k_group_candidates).
 #pragma once
 #include "common.hpp"
+#include "index_kernels.hpp"
 #include "sort.hpp"
 
 namespace culsh {
 
 constexpr int kHcap = 8192;   // LDS hash slots per query
 constexpr int kUcap = 4096;   // unique candidates per query (LDS path)
-constexpr int kScap = 1024;   // candidates re-evaluated in FP64 per query (LDS path)
+constexpr int kScap = 1024;   // candidates re-evaluated in FP64 per query (LDS paths)
 constexpr int kScapG = 4096;  // same, global path
 constexpr int kYcap = 1024;   // query dimensions staged in LDS
+constexpr int kOrderMinQ = 16384;  // batches this large run in locality order
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr double kU32d = 0x1.0p-24;
 
 // FP64 squared L2 in the oracle's order (sequential FMA over d ascending).
 // Rows are fetched 32 floats at a time (8 independent 16-byte loads) ahead
@@ -109,126 +113,82 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
   }
 }
 
-// Where one query's working set lives.
-struct CandBufs {
-  uint32_t* hash;   // 2^hbits slots (open addressing)
-  int hbits;
-  uint32_t* ulist;  // unique candidate ids, capacity ucap
-  unsigned ucap;
-  float* d32;       // FP32 distances, capacity ucap (may alias hash)
-  double* sd;       // FP64 distances of the near-K-th set, capacity scap
-  uint32_t* sid;    // their ids
-  unsigned scap;
-  float* ys;        // query vector staging (kYcap floats, may alias hash), or null
-};
-
-// Processes query q.  Returns false (uniformly) when the query does not fit
-// the buffers; its outputs are then not written.
-__device__ inline bool query_candidates(
-    int q, const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T,
-    int K, const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
-    int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
-    unsigned long long* __restrict__ cu_q, double rel_bound,
-    unsigned long long* __restrict__ n_recheck, const CandBufs& B) {
-  __shared__ uint32_t s_hist[256];
-  __shared__ unsigned long long s_cnu;
-  __shared__ unsigned s_count, s_ovf, s_scount, s_prefix, s_rem, s_misc[4];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const float* y = Qm + (long long)q * D;
-  const uint32_t hslots = 1u << B.hbits, hmask = hslots - 1;
-
-  __syncthreads();  // a previous query's readers of the shared counters are done
-  if (tid == 0) {
-    unsigned long long c = 0;
-    for (int t = 0; t < L; ++t) c += cnu_qt[(long long)q * L + t];
-    s_cnu = c;
-    s_count = 0;
-    s_ovf = 0;
-    s_scount = 0;
-  }
-  for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
-  __syncthreads();
-  const unsigned long long cnu = s_cnu;
-
-  // Union of matched buckets (cuLSH_SearchTables.cu:321-364) with dedup: a
-  // wave takes 64 (table, probe) entries, prefix-sums their bucket sizes and
-  // spreads the member ids over its lanes.
-  if (cnu > 0) {
-    const int LT = L * T;
-    for (int e0 = w * 64; e0 < LT; e0 += 256) {
-      const int e = e0 + lane;
-      uint32_t st = 0, sz = 0;
-      if (e < LT) {
-        const int t = e / T, p = e - t * T;
-        const int k = qb[(long long)t * Q * T + (long long)q * T + p];
-        if (k >= 0) {
-          st = recs[k].start;
-          sz = recs[k].size;
-        }
-      }
-      uint32_t inc = sz;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t yv = __shfl_up(inc, off);
-        if (lane >= off) inc += yv;
-      }
-      const uint32_t tot = __shfl(inc, 63);
-      for (uint32_t b = 0; b < tot; b += 64) {
-        const uint32_t i = b + lane;
-        int lo = 0, hi = 63;
-        while (lo < hi) {  // first lane whose inclusive prefix exceeds i
-          const int mid = (lo + hi) >> 1;
-          const uint32_t im = __shfl(inc, mid);
-          if (im > i)
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-        const uint32_t st_l = __shfl(st, lo);
-        const uint32_t inc_l = __shfl(inc, lo);
-        const uint32_t sz_l = __shfl(sz, lo);
-        if (i < tot) {
-          const uint32_t id = sids[st_l + (i - (inc_l - sz_l))];
-          uint32_t h = (id * 0x9E3779B1u) >> (32 - B.hbits);
-          for (uint32_t probe = 0;; ++probe) {
-            if (probe == hslots) {
-              s_ovf = 1;
-              break;
-            }
-            const uint32_t old = atomicCAS(&B.hash[h], kEmpty, id);
-            if (old == kEmpty) {
-              const unsigned idx = atomicAdd(&s_count, 1u);
-              if (idx < B.ucap)
-                B.ulist[idx] = id;
-              else
-                s_ovf = 1;
-              break;
-            }
-            if (old == id) break;
-            h = (h + 1) & hmask;
-          }
-        }
+// Calls ins(id) for every member of every bucket query q matched
+// (cuLSH_SearchTables.cu:321-340): a wave takes 64 (table, probe) entries,
+// prefix-sums their bucket sizes and spreads the member ids over its lanes.
+// All threads of the workgroup must call it.
+template <class Ins>
+__device__ inline void for_each_member(int q, int Q, int L, int T, const int* __restrict__ qb,
+                                       const BucketRec* __restrict__ recs,
+                                       const uint32_t* __restrict__ sids, Ins ins) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int LT = L * T;
+  for (int e0 = w * 64; e0 < LT; e0 += 256) {
+    const int e = e0 + lane;
+    uint32_t st = 0, sz = 0;
+    if (e < LT) {
+      const int t = e / T, p = e - t * T;
+      const int k = qb[(long long)t * Q * T + (long long)q * T + p];
+      if (k >= 0) {
+        st = recs[k].start;
+        sz = recs[k].size;
       }
     }
+    uint32_t inc = sz;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t yv = __shfl_up(inc, off);
+      if (lane >= off) inc += yv;
+    }
+    const uint32_t tot = __shfl(inc, 63);
+    for (uint32_t b = 0; b < tot; b += 64) {
+      const uint32_t i = b + lane;
+      int lo = 0, hi = 63;
+      while (lo < hi) {  // first lane whose inclusive prefix exceeds i
+        const int mid = (lo + hi) >> 1;
+        const uint32_t im = __shfl(inc, mid);
+        if (im > i)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const uint32_t st_l = __shfl(st, lo);
+      const uint32_t inc_l = __shfl(inc, lo);
+      const uint32_t sz_l = __shfl(sz, lo);
+      if (i < tot) ins(sids[st_l + (i - (inc_l - sz_l))]);
+    }
   }
-  __syncthreads();
-  if (s_ovf) return false;
-  const int U = (int)s_count;
+}
 
-  // The query vector moves to LDS-side staging (hash slots are free now).
-  const bool y_staged = B.ys != nullptr && D <= kYcap;
-  if (y_staged)
-    for (int d = tid; d < D; d += 256) B.ys[d] = y[d];
-  __syncthreads();
-  if (y_staged)
-    l2_fp32_rows(X, B.ys, D, B.ulist, U, B.d32);
-  else
-    l2_fp32_rows(X, y, D, B.ulist, U, B.d32);
-  __syncthreads();
+__device__ inline unsigned long long query_cnu(int q, int L,
+                                               const unsigned long long* __restrict__ cnu_qt) {
+  unsigned long long c = 0;
+  for (int t = 0; t < L; ++t) c += cnu_qt[(long long)q * L + t];
+  return c;
+}
 
-  // K-th smallest FP32 distance by radix select over the float bits
-  // (non-negative floats order like their bit patterns).
+// Exact top-K of query q over its U candidates ids[0..U) given approximate
+// distances dapx[0..U) with |dapx - d| <= rel * d + abs_err.  Selects the
+// K-th smallest dapx (radix select over the float bits; distances are >= 0),
+// re-evaluates in FP64 (the oracle's order) every candidate whose dapx can
+// still be at or below the true K-th distance, and applies the reference's
+// tie rule to the FP64 values.  Writes K outputs.  Returns false
+// (uniformly) when more than scap candidates need the FP64 pass.  All
+// threads of the workgroup must call it.
+__device__ inline bool finish_topk(int q, const float* __restrict__ X, const float* y, int D,
+                                   int K, int U, const float* dapx, const uint32_t* ids,
+                                   double rel, double abs_err, double* sd, uint32_t* sid,
+                                   unsigned scap, int* __restrict__ out_ids,
+                                   float* __restrict__ out_d,
+                                   unsigned long long* __restrict__ n_recheck) {
+  __shared__ uint32_t s_hist[256];
+  __shared__ unsigned s_scount, s_ovf, s_prefix, s_rem, s_misc[4];
+  const int tid = threadIdx.x;
+  __syncthreads();
+  if (tid == 0) {
+    s_scount = 0;
+    s_ovf = 0;
+  }
   double thr = INFINITY;
   if (U > K) {
     if (tid == 0) {
@@ -241,7 +201,7 @@ __device__ inline bool query_candidates(
       __syncthreads();
       const uint32_t prefix = s_prefix;
       for (int u = tid; u < U; u += 256) {
-        const uint32_t bits = __float_as_uint(B.d32[u]);
+        const uint32_t bits = __float_as_uint(fmaxf(dapx[u], 0.f));
         if ((bits & pmask) == prefix) atomicAdd(&s_hist[(bits >> shiftb) & 255u], 1u);
       }
       __syncthreads();
@@ -258,20 +218,21 @@ __device__ inline bool query_candidates(
       pmask |= 255u << shiftb;
       __syncthreads();
     }
-    const double dk32 = (double)__uint_as_float(s_prefix);
-    thr = dk32 * ((1.0 + rel_bound) / (1.0 - rel_bound)) * (1.0 + 1e-12) + (double)D * 0x1.0p-120;
+    const double dk = (double)__uint_as_float(s_prefix);
+    thr = ((dk + abs_err) * ((1.0 + rel) / (1.0 - rel)) + abs_err) * (1.0 + 1e-12) +
+          (double)D * 0x1.0p-120;
+  } else {
+    __syncthreads();
   }
 
-  // Candidates within the error bound of the K-th distance, re-evaluated in
-  // FP64 in the oracle's operation order.
+  // Candidates that can be at or below the true K-th distance, in FP64.
   for (int u = tid; u < U; u += 256) {
-    if ((double)B.d32[u] <= thr) {
+    if ((double)fmaxf(dapx[u], 0.f) <= thr) {
       const unsigned i = atomicAdd(&s_scount, 1u);
-      if (i < B.scap) {
-        const uint32_t id = B.ulist[u];
-        B.sd[i] = y_staged ? dist64(B.ys, X + (long long)id * D, D)
-                           : dist64(y, X + (long long)id * D, D);
-        B.sid[i] = id;
+      if (i < scap) {
+        const uint32_t id = ids[u];
+        sd[i] = dist64(y, X + (long long)id * D, D);
+        sid[i] = id;
       } else {
         s_ovf = 1;
       }
@@ -280,19 +241,15 @@ __device__ inline bool query_candidates(
   __syncthreads();
   if (s_ovf) return false;
   const int S = (int)s_scount;
-  if (tid == 0) {
-    cnu_q[q] = cnu;
-    cu_q[q] = (unsigned long long)U;
-    if (S > 0) atomicAdd(n_recheck, (unsigned long long)S);
-  }
+  if (tid == 0 && S > 0) atomicAdd(n_recheck, (unsigned long long)S);
   // Candidates at or beyond FLT_MAX are never admitted by the reference's
   // heap (initialised to FLT_MAX, strict <, cuLSH_Kernels.cu:426).
   int P2 = 1;
   while (P2 < S) P2 <<= 1;
   for (int i = tid; i < P2; i += 256) {
-    if (i >= S || B.sd[i] >= (double)FLT_MAX) {
-      B.sd[i] = INFINITY;
-      B.sid[i] = 0xFFFFFFFFu;
+    if (i >= S || sd[i] >= (double)FLT_MAX) {
+      sd[i] = INFINITY;
+      sid[i] = 0xFFFFFFFFu;
     }
   }
   __syncthreads();
@@ -303,14 +260,14 @@ __device__ inline bool query_candidates(
         const int ixj = i ^ jj;
         if (ixj > i) {
           const bool upw = (i & kk) == 0;
-          const double a = B.sd[i], b = B.sd[ixj];
-          const uint32_t ia = B.sid[i], ib = B.sid[ixj];
+          const double a = sd[i], b = sd[ixj];
+          const uint32_t ia = sid[i], ib = sid[ixj];
           const bool gt = (a > b) || (a == b && ia > ib);
           if (gt == upw) {
-            B.sd[i] = b;
-            B.sd[ixj] = a;
-            B.sid[i] = ib;
-            B.sid[ixj] = ia;
+            sd[i] = b;
+            sd[ixj] = a;
+            sid[i] = ib;
+            sid[ixj] = ia;
           }
         }
       }
@@ -326,8 +283,7 @@ __device__ inline bool query_candidates(
   }
   __syncthreads();
   for (int i = tid; i < S; i += 256)
-    if (B.sd[i] != INFINITY && (i + 1 == S || B.sd[i + 1] == INFINITY))
-      s_misc[0] = (unsigned)(i + 1);
+    if (sd[i] != INFINITY && (i + 1 == S || sd[i + 1] == INFINITY)) s_misc[0] = (unsigned)(i + 1);
   __syncthreads();
   const int A = (int)s_misc[0];
   int nout = A < K ? A : K;
@@ -337,25 +293,25 @@ __device__ inline bool query_candidates(
     // keep every candidate below d_K, then the r = K - |below| smallest ids
     // among the d_K-ties whose id is >= id*, where id* is the K-th largest
     // id among all candidates at or below d_K.
-    const double dK = B.sd[K - 1];
+    const double dK = sd[K - 1];
     for (int i = tid; i < A; i += 256) {
-      if (B.sd[i] == dK && (i == 0 || B.sd[i - 1] < dK)) s_misc[1] = (unsigned)i;
-      if (B.sd[i] == dK && (i + 1 == A || B.sd[i + 1] > dK)) s_misc[2] = (unsigned)(i + 1);
+      if (sd[i] == dK && (i == 0 || sd[i - 1] < dK)) s_misc[1] = (unsigned)i;
+      if (sd[i] == dK && (i + 1 == A || sd[i + 1] > dK)) s_misc[2] = (unsigned)(i + 1);
     }
     __syncthreads();
     Lt = (int)s_misc[1];
     const int m0 = (int)s_misc[2];
     if (m0 > K) {
       for (int i = tid; i < m0; i += 256) {
-        const uint32_t me = B.sid[i];
+        const uint32_t me = sid[i];
         int greater = 0;
-        for (int j2 = 0; j2 < m0; ++j2) greater += B.sid[j2] > me;
+        for (int j2 = 0; j2 < m0; ++j2) greater += sid[j2] > me;
         if (greater == K - 1) s_misc[3] = me;
       }
       __syncthreads();
       const uint32_t idstar = s_misc[3];
       for (int i = Lt + tid; i < m0; i += 256)
-        if (B.sid[i] >= idstar && (i == Lt || B.sid[i - 1] < idstar)) s_misc[0] = (unsigned)i;
+        if (sid[i] >= idstar && (i == Lt || sid[i - 1] < idstar)) s_misc[0] = (unsigned)i;
       __syncthreads();
       e0 = (int)s_misc[0];
     } else {
@@ -368,8 +324,8 @@ __device__ inline bool query_candidates(
     float dd = FLT_MAX;
     if (k < nout) {
       const int src = k < Lt ? k : e0 + (k - Lt);
-      id = (int)B.sid[src];
-      dd = (float)B.sd[src];
+      id = (int)sid[src];
+      dd = (float)sd[src];
     }
     out_ids[(long long)q * K + k] = id;
     out_d[(long long)q * K + k] = dd;
@@ -378,17 +334,104 @@ __device__ inline bool query_candidates(
   return true;
 }
 
+// Where one query's working set lives (plain form).
+struct CandBufs {
+  uint32_t* hash;   // 2^hbits slots (open addressing)
+  int hbits;
+  uint32_t* ulist;  // unique candidate ids, capacity ucap
+  unsigned ucap;
+  float* d32;       // FP32 distances, capacity ucap (may alias hash)
+  double* sd;       // FP64 distances of the near-K-th set, capacity scap
+  uint32_t* sid;    // their ids
+  unsigned scap;
+  float* ys;        // query vector staging (kYcap floats, may alias hash), or null
+};
+
+// Plain form, query q.  Returns false (uniformly) when the query does not
+// fit the buffers; its outputs are then not written.
+__device__ inline bool query_candidates(
+    int q, const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T,
+    int K, const int* __restrict__ qb, const BucketRec* __restrict__ recs,
+    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
+    unsigned long long* __restrict__ cu_q, unsigned long long* __restrict__ n_recheck,
+    const CandBufs& B) {
+  __shared__ unsigned long long s_cnu;
+  __shared__ unsigned s_count, s_ovf;
+  const int tid = threadIdx.x;
+  const float* y = Qm + (long long)q * D;
+  const uint32_t hslots = 1u << B.hbits, hmask = hslots - 1;
+
+  __syncthreads();  // a previous query's readers of the shared counters are done
+  if (tid == 0) {
+    s_cnu = query_cnu(q, L, cnu_qt);
+    s_count = 0;
+    s_ovf = 0;
+  }
+  for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
+  __syncthreads();
+  const unsigned long long cnu = s_cnu;
+  if (cnu > 0) {
+    for_each_member(q, Q, L, T, qb, recs, sids, [&](uint32_t id) {
+      uint32_t h = (id * 0x9E3779B1u) >> (32 - B.hbits);
+      for (uint32_t probe = 0;; ++probe) {
+        if (probe == hslots) {
+          s_ovf = 1;
+          break;
+        }
+        const uint32_t old = atomicCAS(&B.hash[h], kEmpty, id);
+        if (old == kEmpty) {
+          const unsigned idx = atomicAdd(&s_count, 1u);
+          if (idx < B.ucap)
+            B.ulist[idx] = id;
+          else
+            s_ovf = 1;
+          break;
+        }
+        if (old == id) break;
+        h = (h + 1) & hmask;
+      }
+    });
+  }
+  __syncthreads();
+  if (s_ovf) return false;
+  const int U = (int)s_count;
+
+  // The query vector moves to staging (the hash slots are free now).
+  const bool y_staged = B.ys != nullptr && D <= kYcap;
+  if (y_staged)
+    for (int d = tid; d < D; d += 256) B.ys[d] = y[d];
+  __syncthreads();
+  if (y_staged)
+    l2_fp32_rows(X, B.ys, D, B.ulist, U, B.d32);
+  else
+    l2_fp32_rows(X, y, D, B.ulist, U, B.d32);
+  // Direct FP32: |d32 - d| <= (D + 3) u d; margin 2 (Higham ch. 3).
+  const double rel = 2.0 * (double)(D + 8) * kU32d;
+  if (!finish_topk(q, X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd, B.sid,
+                   B.scap, out_ids, out_d, n_recheck))
+    return false;
+  if (tid == 0) {
+    cnu_q[q] = cnu;
+    cu_q[q] = (unsigned long long)U;
+  }
+  return true;
+}
+
 // One workgroup per query, all buffers in LDS (~50 KB).  After the union the
 // hash slots hold d32 [kUcap], sd [kScap] (f64), sid [kScap] and the query.
 // Queries that do not fit are appended to overflow[] for
-// k_candidates_global.
+// k_candidates_global.  With `order` (a locality order of the queries),
+// XCD x (which receives workgroups x, x+8, ...) takes the x-th eighth of the
+// order in sequence, so queries sharing candidate rows run on one L2.
 __global__ __launch_bounds__(256) void k_candidates(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
     const int* __restrict__ qb, const BucketRec* __restrict__ recs,
     const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, int* __restrict__ overflow,
-    int* __restrict__ n_overflow, double rel_bound, unsigned long long* __restrict__ n_recheck) {
+    int* __restrict__ n_overflow, unsigned long long* __restrict__ n_recheck,
+    const uint32_t* __restrict__ order) {
   __shared__ uint32_t s_hash[kHcap];
   __shared__ uint32_t s_u[kUcap];
   static_assert(kUcap + 3 * kScap + kYcap <= kHcap, "post-union LDS layout");
@@ -402,9 +445,15 @@ __global__ __launch_bounds__(256) void k_candidates(
   B.sid = s_hash + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_hash + kUcap + 3 * kScap);
-  const int q = blockIdx.x;
+  int q = blockIdx.x;
+  if (order) {
+    const int per = (int)(gridDim.x >> 3);  // grid is a multiple of 8
+    const int pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (pos >= Q) return;
+    q = (int)order[pos];
+  }
   if (!query_candidates(q, X, Qm, Q, D, L, T, K, qb, recs, sids, cnu_qt, out_ids, out_d, cnu_q,
-                        cu_q, rel_bound, n_recheck, B)) {
+                        cu_q, n_recheck, B)) {
     if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1)] = q;
   }
 }
@@ -415,9 +464,7 @@ __global__ __launch_bounds__(256) void k_max_cnu(const int* __restrict__ list, i
                                                  int L, unsigned long long* __restrict__ out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_list) return;
-  unsigned long long c = 0;
-  for (int t = 0; t < L; ++t) c += cnu_qt[(long long)list[i] * L + t];
-  atomicMax(out, c);
+  atomicMax(out, query_cnu(list[i], L, cnu_qt));
 }
 
 // The overflowed queries: workgroup b takes list[b], list[b + G], ...; its
@@ -430,7 +477,7 @@ __global__ __launch_bounds__(256) void k_candidates_global(
     const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, const int* __restrict__ list, int n_list,
-    uint32_t* __restrict__ scratch, int hbits, unsigned ucap, double rel_bound,
+    uint32_t* __restrict__ scratch, int hbits, unsigned ucap,
     unsigned long long* __restrict__ n_recheck, int* __restrict__ err) {
   __shared__ double s_sd[kScapG];
   __shared__ uint32_t s_sid[kScapG];
@@ -448,8 +495,252 @@ __global__ __launch_bounds__(256) void k_candidates_global(
   B.ys = s_y;
   for (int i = blockIdx.x; i < n_list; i += gridDim.x) {
     if (!query_candidates(list[i], X, Qm, Q, D, L, T, K, qb, recs, sids, cnu_qt, out_ids, out_d,
-                          cnu_q, cu_q, rel_bound, n_recheck, B)) {
+                          cnu_q, cu_q, n_recheck, B)) {
       if (threadIdx.x == 0) atomicOr(err, 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Precision form: grouped queries, norm-expansion distances on MFMA.
+// ---------------------------------------------------------------------------
+constexpr int kGroup = 16;   // queries per group (the MFMA tile's M)
+constexpr int kGRows = 64;   // union rows per MFMA step (4 waves x 16)
+constexpr int kGDc = 64;     // dimensions staged per MFMA step
+
+// Coarse locality key of each query: floor((a_m . q - b_m) / (4 W)) of the
+// first min(6, M) hashes of table 0, ten bits each.  Only orders queries so
+// that queries sharing candidates are grouped; results do not depend on it.
+__global__ __launch_bounds__(256) void k_locality_key(const float* __restrict__ Qm, int Q, int D,
+                                                      int M, const float* __restrict__ Aorig,
+                                                      const float* __restrict__ bj, float W,
+                                                      uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= Q) return;
+  const float* y = Qm + (long long)q * D;
+  uint64_t key = 0;
+  const int mm = M < 6 ? M : 6;
+  for (int m = 0; m < 6; ++m) {
+    int c = 0;
+    if (m < mm) {
+      const float* a = Aorig + (long long)m * D;
+      float s = 0.f;
+      for (int d = 0; d < D; ++d) s = fmaf(y[d], a[d], s);
+      c = (int)floorf((s - bj[m]) / (4.f * W));
+    }
+    key = (key << 10) | (uint64_t)((c + 512) & 1023);
+  }
+  keys[q] = key;
+  vals[q] = (uint32_t)q;
+}
+
+// Sum of C_nu over each group of kGroup consecutive queries of `order`; the
+// largest goes to *out.
+__global__ __launch_bounds__(256) void k_group_cnu(const uint32_t* __restrict__ order, int Q, int L,
+                                                   const unsigned long long* __restrict__ cnu_qt,
+                                                   unsigned long long* __restrict__ out) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int ng = (Q + kGroup - 1) / kGroup;
+  if (g >= ng) return;
+  unsigned long long s = 0;
+  for (int i = g * kGroup; i < min(Q, (g + 1) * kGroup); ++i) s += query_cnu(order[i], L, cnu_qt);
+  atomicMax(out, s);
+}
+
+// One workgroup per group of up to 16 queries (taken in `order`), groups
+// strided over the grid.  Per-workgroup global scratch of `region` words:
+//   hash ids [2^hbits], complemented hash masks [2^hbits] (bit g clear =
+//   query g has the id), union slots [qcap], per-query approximate distances
+//   [qcap] and ids [qcap], union ids [qcap] and masks [qcap]
+// where qcap >= the group's summed C_nu.  The scratch starts as all-ones
+// bytes (empty ids, empty masks) and each group restores the slots it used,
+// so no per-group clearing pass is needed.  Each union row is read once per
+// d-chunk and multiplied against the group's queries on
+// v_mfma_f32_16x16x4_f32:  d~ = |q|^2 + |x|^2 - 2 q.x  (FP32), with
+// |d~ - d| <= 2 (D+4) u (|q| + xmax)^2 (Higham ch. 3, margin 2).  Then each
+// query runs finish_topk on its own candidates.  Queries that need more
+// FP64 rechecks than the LDS holds go to overflow[].
+__global__ __launch_bounds__(256) void k_group_candidates(
+    const float* __restrict__ X, const float* __restrict__ xn2, float xmax,
+    const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
+    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
+    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    const uint32_t* __restrict__ order, int* __restrict__ out_ids, float* __restrict__ out_d,
+    unsigned long long* __restrict__ cnu_q, unsigned long long* __restrict__ cu_q,
+    uint32_t* __restrict__ scratch, size_t region, int hbits, unsigned qcap,
+    int* __restrict__ overflow, int* __restrict__ n_overflow,
+    unsigned long long* __restrict__ n_recheck) {
+  __shared__ float s_q[kGroup][kGDc + 4];
+  __shared__ float s_x[kGRows][kGDc + 4];
+  __shared__ uint32_t s_rid[kGRows];
+  __shared__ double s_sd[kScap];
+  __shared__ uint32_t s_sid[kScap];
+  __shared__ float s_qn[kGroup], s_qnorm[kGroup];
+  __shared__ int s_qid[kGroup];
+  __shared__ unsigned s_qoff[kGroup], s_qcnt[kGroup], s_ucount, s_ovf;
+  __shared__ unsigned long long s_cnu[kGroup];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t hslots = 1u << hbits, hmask = hslots - 1;
+  uint32_t* hid = scratch + (size_t)blockIdx.x * region;
+  uint32_t* hmk = hid + hslots;
+  uint32_t* uslot = hmk + hslots;
+  float* qd = reinterpret_cast<float*>(uslot + qcap);
+  uint32_t* qi = uslot + 2ull * qcap;
+  uint32_t* uid = uslot + 3ull * qcap;  // union ids, resolved once
+  uint32_t* umk = uslot + 4ull * qcap;  // their membership masks
+  const bool vec = (D & 3) == 0;
+  const int ng = (Q + kGroup - 1) / kGroup;
+
+  for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+    const int g0 = grp * kGroup;
+    const int gs = min(kGroup, Q - g0);
+    __syncthreads();
+    if (tid < kGroup) {
+      s_qid[tid] = tid < gs ? (int)order[g0 + tid] : -1;
+      s_cnu[tid] = tid < gs ? query_cnu((int)order[g0 + tid], L, cnu_qt) : 0ull;
+      s_qcnt[tid] = 0;
+    }
+    if (tid == 0) {
+      s_ucount = 0;
+      s_ovf = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned off = 0;
+      for (int g = 0; g < kGroup; ++g) {
+        s_qoff[g] = off;
+        off += (unsigned)s_cnu[g];
+      }
+    }
+    // Query norms (FP32) for the expansion and its error bound.
+    if (w == 0 && lane < kGroup) {
+      float s = 0.f;
+      if (lane < gs) {
+        const float* y = Qm + (long long)s_qid[lane] * D;
+        for (int d = 0; d < D; ++d) s = fmaf(y[d], y[d], s);
+      }
+      s_qn[lane] = s;
+      s_qnorm[lane] = sqrtf(s);
+    }
+    // Union of the group's candidates; hash slot masks record membership.
+    for (int g = 0; g < gs; ++g) {
+      const int q = s_qid[g];
+      for_each_member(q, Q, L, T, qb, recs, sids, [&](uint32_t id) {
+        uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
+        for (uint32_t probe = 0;; ++probe) {
+          if (probe == hslots) {
+            s_ovf = 1;
+            break;
+          }
+          const uint32_t old = atomicCAS(&hid[h], kEmpty, id);
+          if (old == kEmpty || old == id) {
+            if (old == kEmpty) uslot[atomicAdd(&s_ucount, 1u)] = h;
+            const uint32_t om = atomicAnd(&hmk[h], ~(1u << g));
+            if (om & (1u << g)) atomicAdd(&s_qcnt[g], 1u);
+            break;
+          }
+          h = (h + 1) & hmask;
+        }
+      });
+    }
+    __syncthreads();
+    const int UG = (int)s_ucount;
+    if (s_ovf) {  // cannot happen with qcap >= summed C_nu; fail safe
+      for (int r = tid; r < UG; r += 256) {
+        hid[uslot[r]] = kEmpty;
+        hmk[uslot[r]] = 0xFFFFFFFFu;
+      }
+      if (tid < gs) overflow[atomicAdd(n_overflow, 1)] = s_qid[tid];
+      continue;
+    }
+    unsigned cu[kGroup];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) cu[g] = s_qcnt[g];
+    for (int r = tid; r < UG; r += 256) {
+      const uint32_t slot = uslot[r];
+      uid[r] = hid[slot];
+      umk[r] = ~hmk[slot];
+    }
+    __syncthreads();
+    if (tid < kGroup) s_qcnt[tid] = 0;
+    __syncthreads();
+
+    // Union rows in chunks of 64: q.x on MFMA, then d~ for member pairs.
+    for (int r0 = 0; r0 < UG; r0 += kGRows) {
+      if (tid < kGRows) s_rid[tid] = r0 + tid < UG ? uid[r0 + tid] : uid[r0];
+      __syncthreads();
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int dc = 0; dc < D; dc += kGDc) {
+        if (vec) {
+          // 16 float4 per row chunk, 4 per thread, loads issued together.
+          for (int e = tid; e < kGRows * (kGDc / 4); e += 256) {
+            const int r = e >> 4, c4 = e & 15;
+            const int d = dc + 4 * c4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (d < D) v = *reinterpret_cast<const float4*>(X + (long long)s_rid[r] * D + d);
+            *reinterpret_cast<float4*>(&s_x[r][4 * c4]) = v;
+          }
+        } else {
+          for (int e = tid; e < kGRows * kGDc; e += 256) {
+            const int r = e / kGDc, c = e % kGDc;
+            const int d = dc + c;
+            s_x[r][c] = d < D ? X[(long long)s_rid[r] * D + d] : 0.f;
+          }
+        }
+        for (int e = tid; e < kGroup * kGDc; e += 256) {
+          const int g = e / kGDc, c = e % kGDc;
+          const int d = dc + c;
+          s_q[g][c] = (g < gs && d < D) ? Qm[(long long)s_qid[g] * D + d] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kGDc; kk += 4) {
+          const float a = s_q[lane & 15][kk + (lane >> 4)];
+          const float b = s_x[w * 16 + (lane & 15)][kk + (lane >> 4)];
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+      }
+      // lane: union row r0 + 16w + (lane & 15), queries 4 * (lane >> 4) + i
+      const int r = r0 + w * 16 + (lane & 15);
+      if (r < UG) {
+        const uint32_t id = s_rid[w * 16 + (lane & 15)], mk = umk[r];
+        const float xq = xn2[id];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int g = (lane >> 4) * 4 + i;
+          if (g < gs && ((mk >> g) & 1u)) {
+            const float dap = s_qn[g] + xq - 2.f * acc[i];
+            const unsigned pos = s_qoff[g] + atomicAdd(&s_qcnt[g], 1u);
+            qd[pos] = dap;
+            qi[pos] = id;
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    for (int g = 0; g < gs; ++g) {
+      const int q = s_qid[g];
+      const double qb2 = (double)s_qnorm[g] + (double)xmax;
+      const double abs_err = 2.0 * (double)(D + 4) * kU32d * qb2 * qb2 * (1.0 + 1e-6);
+      const bool ok = finish_topk(q, X, Qm + (long long)q * D, D, K, (int)cu[g], qd + s_qoff[g],
+                                  qi + s_qoff[g], 0.0, abs_err, s_sd, s_sid, kScap, out_ids,
+                                  out_d, n_recheck);
+      if (tid == 0) {
+        if (ok) {
+          cnu_q[q] = s_cnu[g];
+          cu_q[q] = cu[g];
+        } else {
+          overflow[atomicAdd(n_overflow, 1)] = q;
+        }
+      }
+    }
+    // Restore the used hash slots for the next group.
+    for (int r = tid; r < UG; r += 256) {
+      hid[uslot[r]] = kEmpty;
+      hmk[uslot[r]] = 0xFFFFFFFFu;
     }
   }
 }

@@ -374,10 +374,16 @@ int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
   const long long tiles = nrt * (Jp / kPjCols);
   const long long grid = (tiles + 7) / 8 * 8;
   if (grid >= (1ll << 31)) return fail(CULSH_E_LIMIT, "projection grid too large");
-  hipLaunchKernelGGL(k_project_f32, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
-                     (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
-                     (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p, cnt.p,
-                     cap, err.p);
+  if (ht->form == CULSH_FORM_PRECISION)
+    hipLaunchKernelGGL(k_project_bf16x3, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
+                       (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
+                       (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p,
+                       cnt.p, cap, err.p);
+  else
+    hipLaunchKernelGGL(k_project_f32, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
+                       (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
+                       (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p,
+                       cnt.p, cap, err.p);
   unsigned long long count = 0;
   HIP_TRY(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -625,7 +631,8 @@ unsigned long long culsh_ht_rechecked(const culsh_ht* ht) { return ht ? ht->rech
 
 int culsh_ht_set_form(culsh_ht* ht, int form) {
   if (!ht) return fail(CULSH_E_ARG, "null HashTables");
-  if (form != CULSH_FORM_PLAIN) return fail(CULSH_E_LIMIT, "form %d not available", form);
+  if (form != CULSH_FORM_PLAIN && form != CULSH_FORM_PRECISION)
+    return fail(CULSH_E_ARG, "unknown form %d", form);
   ht->form = form;
   return CULSH_OK;
 }
@@ -853,6 +860,15 @@ struct culsh_st {
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt, dmax;
   DevBuf<uint32_t> dscratch;  // working sets of overflowed queries
   DevBuf<int> dnov;
+  // Precision form: locality order of the queries and group working sets.
+  DevBuf<uint64_t> dlk0, dlk1;
+  DevBuf<uint32_t> dlv0, dlv1, dlscr, dgscratch;
+  bool gscratch_clean = false;  // group hash slots hold their empty state
+  // |x|^2 and max |x| of the dataset the last precision search used.
+  DevBuf<float> dxn2;
+  DevBuf<unsigned> dxmax;
+  const float* xn2_for = nullptr;
+  float xmax = 0.f;
   std::vector<int> ids;
   std::vector<float> dists;
   std::vector<std::vector<int>> qbuckets;
@@ -906,12 +922,11 @@ int run_overflow(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_i
   const int G = (int)std::max<size_t>(1, std::min<size_t>((size_t)nov, budget / region));
   TRY(st->dscratch.ensure(region * G));
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
-  const double rel = 2.0 * (double)(D + 8) * 0x1.0p-24;
   hipLaunchKernelGGL(k_candidates_global, dim3(G), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
                      (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
                      (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
                      d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p, nov,
-                     st->dscratch.p, hbits, ucap, rel, st->dcnt.p, st->dnov.p);
+                     st->dscratch.p, hbits, ucap, st->dcnt.p, st->dnov.p);
   HIP_TRY(hipGetLastError());
   int err = 0;
   HIP_TRY(hipMemcpyAsync(&err, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
@@ -921,6 +936,82 @@ int run_overflow(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_i
                 "a query has more than %d candidates within the FP32 error bound of its K-th "
                 "distance",
                 kScapG);
+  return CULSH_OK;
+}
+
+// Locality order of the queries: a stable radix sort of coarse hash codes of
+// table 0, so queries likely to share buckets become neighbours.
+int locality_order(culsh_st* st, const float* dQ, int Q, hipStream_t s, const uint32_t** order) {
+  culsh_ht* ht = st->ht;
+  TRY(st->dlk0.ensure(Q));
+  TRY(st->dlk1.ensure(Q));
+  TRY(st->dlv0.ensure(Q));
+  TRY(st->dlv1.ensure(Q));
+  TRY(st->dlscr.ensure(rs_scratch_elems(Q) + 1));
+  hipLaunchKernelGGL(k_locality_key, dim3(grid_for(Q)), dim3(256), 0, s, dQ, Q, ht->D, ht->M,
+                     (const float*)ht->dAorig.p, (const float*)ht->dbj.p, ht->W, st->dlk0.p,
+                     st->dlv0.p);
+  uint64_t* kp = st->dlk0.p;
+  uint32_t* vp = st->dlv0.p;
+  radix_sort_pairs(&kp, &vp, st->dlk1.p, st->dlv1.p, Q, 0, 64, st->dlscr.p, s);
+  *order = vp;
+  return CULSH_OK;
+}
+
+// Precision form candidate stage: |x|^2 of the dataset (cached per dataset
+// pointer), locality order of the queries, group working sets sized from the
+// largest group's summed C_nu (one host synchronisation), then
+// k_group_candidates.  Queries it cannot finish go to the overflow list.
+int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids,
+                         float* d_dists, hipStream_t s) {
+  culsh_ht* ht = st->ht;
+  const int L = ht->L, D = ht->D, T = st->T, K = st->K;
+  const long long N = ht->N;
+  if (st->xn2_for != dX) {
+    TRY(st->dxn2.ensure(N));
+    TRY(st->dxmax.ensure(1));
+    HIP_TRY(hipMemsetAsync(st->dxmax.p, 0, 4, s));
+    hipLaunchKernelGGL(k_row_sqnorms, dim3(grid_for(N, 16)), dim3(256), 0, s, dX, N, D,
+                       st->dxn2.p, st->dxmax.p);
+    unsigned bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, st->dxmax.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(&st->xmax, &bits, 4);
+    st->xn2_for = dX;
+  }
+  const uint32_t* order = nullptr;
+  TRY(locality_order(st, dQ, Q, s, &order));
+  const int ng = (Q + kGroup - 1) / kGroup;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dmax.p, 1);
+  hipLaunchKernelGGL(k_group_cnu, dim3(grid_for(ng)), dim3(256), 0, s, order, Q, L,
+                     (const unsigned long long*)st->dcnu_qt.p, st->dmax.p);
+  unsigned long long gmax = 0;
+  HIP_TRY(hipMemcpyAsync(&gmax, st->dmax.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (gmax >= (1ull << 29))
+    return fail(CULSH_E_LIMIT, "a query group has %llu candidates (limit 2^29)", gmax);
+  const unsigned qcap = (unsigned)std::max<unsigned long long>(gmax, 1);
+  const int hbits = std::max(8, ceil_log2_host(2ll * qcap));
+  const size_t region = 2 * (1ull << hbits) + 5ull * qcap;
+  const size_t budget = (4ull << 30) / 4;
+  const int G = (int)std::max<size_t>(
+      1, std::min<size_t>(std::min<size_t>((size_t)ng, budget / region), 4096));
+  if (st->dgscratch.n < region * G) {
+    TRY(st->dgscratch.exact(region * G));
+    st->gscratch_clean = false;
+  }
+  if (!st->gscratch_clean) {
+    // Empty hash ids and complemented masks are all-ones words.
+    HIP_TRY(hipMemsetAsync(st->dgscratch.p, 0xFF, st->dgscratch.n * 4, s));
+    st->gscratch_clean = true;
+  }
+  hipLaunchKernelGGL(k_group_candidates, dim3(G), dim3(256), 0, s, dX,
+                     (const float*)st->dxn2.p, st->xmax, dQ, Q, D, L, T, K,
+                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
+                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                     order, d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, st->dgscratch.p, region,
+                     hbits, qcap, st->dov.p, st->dnov.p, st->dcnt.p);
+  HIP_TRY(hipGetLastError());
   return CULSH_OK;
 }
 
@@ -1001,11 +1092,23 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
 #undef CULSH_PROBE
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[1], s));
-  const double rel = 2.0 * (double)(D + 8) * 0x1.0p-24;
-  hipLaunchKernelGGL(k_candidates, dim3(Q), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
-                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
-                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p, d_ids,
-                     d_dists, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p, rel, st->dcnt.p);
+  if (st->form == CULSH_FORM_PRECISION) {
+    TRY(run_group_candidates(st, dQ, dX, Q, d_ids, d_dists, s));
+  } else {
+    // Large batches run in locality order, each XCD taking a contiguous run
+    // of it, so queries that share candidate rows share an L2.
+    const uint32_t* order = nullptr;
+    unsigned grid = (unsigned)Q;
+    if (Q >= kOrderMinQ) {
+      TRY(locality_order(st, dQ, Q, s, &order));
+      grid = (unsigned)((Q + 7) / 8 * 8);
+    }
+    hipLaunchKernelGGL(k_candidates, dim3(grid), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
+                       (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
+                       (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                       d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p,
+                       st->dcnt.p, order);
+  }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
   HIP_TRY(hipGetLastError());
   int nov = 0;
@@ -1039,7 +1142,8 @@ int culsh_st_reset(culsh_st* st, culsh_ht* ht, int K, int T) {
 
 int culsh_st_set_form(culsh_st* st, int form) {
   if (!st) return fail(CULSH_E_ARG, "null SearchTables");
-  if (form != CULSH_FORM_PLAIN) return fail(CULSH_E_LIMIT, "form %d not available", form);
+  if (form != CULSH_FORM_PLAIN && form != CULSH_FORM_PRECISION)
+    return fail(CULSH_E_ARG, "unknown form %d", form);
   st->form = form;
   return CULSH_OK;
 }

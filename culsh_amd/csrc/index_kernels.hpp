@@ -30,6 +30,27 @@ __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, 
   if (n < N && lig == 0) nrm[n] = (float)(sqrt(acc) * (1.0 + 0x1.0p-20));
 }
 
+// FP32 squared norm of each row (the norm expansion's |x|^2) and the largest
+// row norm (as float bits; norms are non-negative so integer max orders
+// them).
+__global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X, long long N,
+                                                     int D, float* __restrict__ xn2,
+                                                     unsigned* __restrict__ max_norm_bits) {
+  const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
+  const long long n = (long long)blockIdx.x * 16 + g;
+  float acc = 0.f;
+  if (n < N) {
+    const float* x = X + n * D;
+    for (int d = lig; d < D; d += 16) acc = fmaf(x[d], x[d], acc);
+  }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+  if (n < N && lig == 0) {
+    xn2[n] = acc;
+    atomicMax(max_norm_bits, __float_as_uint(sqrtf(acc) * (1.0f + 1e-4f)));
+  }
+}
+
 // Records projection (n, j) for FP64 re-evaluation.
 __device__ inline void push_recheck(long long n, int j, unsigned long long* __restrict__ list,
                                     unsigned long long* __restrict__ count,
@@ -137,6 +158,102 @@ __global__ __launch_bounds__(256) void k_project_f32(
       const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
       if (n >= N) continue;
       project_epilogue(acc[c][i], n, j, N, M, bb, W, ka * (double)nx[n] * na, codes, list, count,
+                       cap, err);
+    }
+  }
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// float -> bf16, round to nearest even (finite inputs).
+__device__ inline uint16_t f2bf_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf_bits2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Splits x into hi + lo bf16 parts: |x - hi - lo| <= 2^-16 |x|.
+__device__ inline void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+  hi = f2bf_bits(x);
+  lo = f2bf_bits(x - bf_bits2f(hi));
+}
+
+// Precision form: BF16x3 split-precision projection on MFMA
+// (v_mfma_f32_16x16x32_bf16), x.a ~ xh.ah + xh.al + xl.ah with FP32
+// accumulation.  Same tiling and epilogue as k_project_f32.  Error bound:
+// each term is within 3.01 * 2^-16 |x_d a_d| of x_d a_d (dropped xl.al and
+// the split residuals) and the 3D-term FP32 accumulation adds
+// 3.1 D 2^-24 sum |x_d a_d|; with sum |x_d a_d| <= ||x|| ||a_j|| and a
+// factor 2 margin.
+__global__ __launch_bounds__(256) void k_project_bf16x3(
+    const float* __restrict__ X, long long N, int D, const float* __restrict__ Ad, int Jp, int J,
+    int M, const float* __restrict__ bj, const float* __restrict__ naj,
+    const float* __restrict__ nx, float W, int16_t* __restrict__ codes,
+    unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
+    unsigned long long cap, int* __restrict__ err) {
+  __shared__ float Xs[kPjRows][kPjKc + 1];
+  __shared__ float As[kPjKc][kPjCols + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nct = Jp / kPjCols;
+  const long long nrt = (N + kPjRows - 1) / kPjRows;
+  long long rt;
+  int ct;
+  if (!xcd_tile(nct, nrt * nct, rt, ct)) return;
+  const long long n0 = rt * kPjRows;
+  const int j0 = ct * kPjCols;
+  floatx4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int kb = (lane >> 4) * 8;  // this lane's 8 consecutive k of the 32
+  for (int d0 = 0; d0 < D; d0 += kPjKc) {
+    for (int e = tid; e < kPjRows * kPjKc; e += 256) {
+      const int r = e / kPjKc, c = e % kPjKc;
+      const long long n = n0 + r;
+      const int d = d0 + c;
+      Xs[r][c] = (n < N && d < D) ? X[n * D + d] : 0.f;
+    }
+    for (int e = tid; e < kPjKc * kPjCols; e += 256) {
+      const int r = e / kPjCols, c = e % kPjCols;
+      const int d = d0 + r;
+      As[r][c] = (d < D) ? Ad[(long long)d * Jp + j0 + c] : 0.f;
+    }
+    __syncthreads();
+    bf16x8 xh, xl;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      uint16_t h, l;
+      split_bf16(Xs[w * 16 + (lane & 15)][kb + i], h, l);
+      xh[i] = __builtin_bit_cast(__bf16, h);
+      xl[i] = __builtin_bit_cast(__bf16, l);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bf16x8 ah, al;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        uint16_t h, l;
+        split_bf16(As[kb + i][c * 16 + (lane & 15)], h, l);
+        ah[i] = __builtin_bit_cast(__bf16, h);
+        al[i] = __builtin_bit_cast(__bf16, l);
+      }
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, ah, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, al, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, ah, acc[c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const double kb3 = 2.0 * (3.01 * 0x1.0p-16 + 3.1 * (double)D * 0x1.0p-24);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = j0 + c * 16 + (lane & 15);
+    if (j >= J) continue;
+    const float bb = bj[j];
+    const double na = (double)naj[j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
+      if (n >= N) continue;
+      project_epilogue(acc[c][i], n, j, N, M, bb, W, kb3 * (double)nx[n] * na, codes, list, count,
                        cap, err);
     }
   }

@@ -93,6 +93,19 @@ def test_large_candidate_sets_use_global_path():
         assert_search_equal(g, o)
 
 
+def test_locality_ordered_large_batch():
+    """Batches of 16384+ queries run the candidate kernel in locality order
+    (each XCD takes a contiguous run of it); Q = 16387 leaves a partial
+    last run."""
+    print("locality-ordered batch Q=16387", flush=True)
+    X = wl.mixture(20000, 16, 40, 0.2, 101)
+    Y = wl.queries_near(X, 16387, 0.05, 102)
+    A, b = wl.projection(4, 16, 6, 1.0, 103)
+    g = run_gpu(X, Y, A, b, 4, 6, 1.0, 5, 8)
+    o = run_oracle(X, Y, A, b, 4, 6, 1.0, 5, 8)
+    assert_search_equal(g, o)
+
+
 def test_exact_delta_ties():
     """Duplicated projection columns give exactly tied deltas: the delta order
     then follows the reference heap's mechanics."""
@@ -140,6 +153,44 @@ def test_config3_slice():
     g = run_gpu(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
     o = run_oracle(X, Y, A, b, cfg.L, cfg.M, cfg.W, cfg.K, cfg.T)
     assert_search_equal(g, o)
+
+
+@pytest.mark.parametrize("case", ["small_T8", "small_T40_K33", "ties", "c1", "c3_slice",
+                                  "large"])
+def test_precision_form_matches_oracle(case):
+    """Precision form: BF16x3 MFMA projection for the index, grouped
+    norm-expansion MFMA distances for the search; FP64 rechecks keep every
+    result bit-exact."""
+    print("precision", case, flush=True)
+    if case in ("small_T8", "small_T40_K33"):
+        X, Y, A, b = _small(91)
+        L, M, W = 4, 6, 1.0
+        K, T = (5, 8) if case == "small_T8" else (33, 40)
+    elif case == "ties":
+        rng = np.random.default_rng(92)
+        X = rng.integers(0, 4, size=(3000, 8)).astype(np.float32)
+        Y = rng.integers(0, 4, size=(200, 8)).astype(np.float32)
+        L, M, W, K, T = 6, 4, 3.0, 16, 12
+        A, b = wl.projection(L, 8, M, W, 93)
+    elif case == "c1":
+        cfg = wl.CONFIGS["c1"]
+        X, Y, A, b = wl.make_data(cfg)
+        L, M, W, K, T = cfg.L, cfg.M, cfg.W, cfg.K, cfg.T
+    elif case == "c3_slice":
+        cfg = wl.CONFIGS["c3"]
+        X, Y, A, b = wl.make_data(cfg, N=400_000, Q=1_000)
+        L, M, W, K, T = cfg.L, cfg.M, cfg.W, cfg.K, cfg.T
+    else:
+        X = wl.mixture(30000, 16, 3, 0.3, 81)
+        Y = wl.queries_near(X, 40, 0.05, 82)
+        A, b = wl.projection(3, 16, 3, 8.0, 83)
+        L, M, W, K, T = 3, 3, 8.0, 1500, 2
+    g = run_gpu(X, Y, A, b, L, M, W, K, T, form=1)
+    o = run_oracle(X, Y, A, b, L, M, W, K, T)
+    assert_buckets_equal(g["ht"], o["index"], L)
+    assert_search_equal(g, o)
+    print("  index rechecks", g["ht"].getRechecked(), "search rechecks",
+          g["st"].getRechecked(), flush=True)
 
 
 def test_device_api_matches_host_api():
