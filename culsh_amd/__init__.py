@@ -4,6 +4,8 @@ A drop-in for the cuLSH HashTables/SearchTables path: hand-written gfx950 HIP
 kernels in ``lib/libculsh.so`` behind the C ABI of ``include/culsh_c.h``,
 with this package as the Python mirror of the reference classes.
 """
-from .lsh import FORM_PLAIN, FORM_PRECISION, HashTables, SearchTables  # noqa: F401
+from .lsh import (FORM_PLAIN, FORM_PRECISION, Batch, HashTables, SearchTables,  # noqa: F401
+                  layout_requests)
 
-__all__ = ["HashTables", "SearchTables", "FORM_PLAIN", "FORM_PRECISION"]
+__all__ = ["HashTables", "SearchTables", "Batch", "layout_requests", "FORM_PLAIN",
+           "FORM_PRECISION"]

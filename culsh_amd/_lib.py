@@ -20,6 +20,16 @@ class Times(ctypes.Structure):
                 ("unique", c_float), ("distances", c_float), ("knn", c_float)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("Q", c_int), ("K", c_int), ("T", c_int), ("reserved", c_int),
+                ("q_row", ctypes.c_longlong), ("out_offset", ctypes.c_longlong)]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("probe_queries", c_int), ("probe_tables", c_int), ("tables_per_item", c_int),
+                ("rows_in_flight", c_int), ("wgs_per_cu", c_int)]
+
+
 # name -> (restype, argtypes); every symbol include/culsh_c.h declares.
 SIGNATURES = {
     "culsh_last_error": (c_char_p, []),
@@ -66,6 +76,16 @@ SIGNATURES = {
     "culsh_st_query_cu": (c_vp, [c_vp]),
     "culsh_st_rechecked": (c_ull, [c_vp]),
     "culsh_st_kernel_ms": (c_int, [c_vp, P(c_float), P(c_float)]),
+    "culsh_bt_create": (c_int, [c_vp, P(c_vp)]),
+    "culsh_bt_destroy": (None, [c_vp]),
+    "culsh_bt_set_schedule": (c_int, [c_vp, P(Schedule)]),
+    "culsh_bt_get_schedule": (c_int, [c_vp, P(Schedule)]),
+    "culsh_bt_search_device": (c_int, [c_vp, P(Request), c_int, c_vp, c_vp, c_vp, c_vp]),
+    "culsh_bt_wait": (c_int, [c_vp]),
+    "culsh_bt_last_ms": (c_int, [c_vp, P(c_float)]),
+    "culsh_bt_rechecked": (c_ull, [c_vp]),
+    "culsh_bt_autotune": (c_int, [c_vp, P(Request), c_int, c_vp, c_vp, c_vp, c_int, P(c_float),
+                                  P(c_float)]),
 }
 
 FORM_PLAIN = 0

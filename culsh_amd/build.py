@@ -24,6 +24,9 @@ HIP_FLAGS = [
     # Packed FP32 (v_pk_*_f32) stays off: ordinary float code only.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # __launch_bounds__ occupancy targets are hints; paths whose registers do
+    # not fit them run at the occupancy they get.
+    "-Wno-pass-failed",
 ]
 CXX_FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
              "-std=c++17", "-Wall"]

@@ -1,0 +1,445 @@
+// Host side of the batching form (culsh_bt_*): request layout, capacities,
+// the captured graph and the schedule autotuner.  Part of libculsh's single
+// translation unit (included by culsh.hip after the index and search code,
+// whose helpers and index state it uses).
+#pragma once
+
+struct culsh_bt {
+  culsh_ht* ht = nullptr;
+  culsh_schedule sched{32, 1, 1, 4, 2};
+  hipStream_t cap = nullptr;  // capture stream
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int ncu = 0;
+  // Capacities of the device buffers the graph was captured with.
+  int Qcap = 0, Rcap = 0, Bcap = 0;
+  long long Pcap = 0;
+  size_t off_cls = 0, off_req = 0, packed = 0;
+  DevBuf<unsigned char> dpack;  // BatchHdr | BatchClass[Rcap] | BatchReq[Rcap]
+  unsigned char* hpack = nullptr;  // pinned host image of dpack
+  BatchHdr* hres = nullptr;        // pinned: header after the batch
+  DevBuf<BatchQuery> dbq;
+  DevBuf<unsigned> dblk;
+  DevBuf<int> dqb, dovl;
+  DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q;
+  DevBuf<uint32_t> dscr;
+  static constexpr int kOvGrid = 16;
+  static constexpr int kHbitsMax = 21;
+  static constexpr unsigned kUcapMax = 1u << 20;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  culsh_schedule graph_sched{0, 0, 0, 0, 0};
+  bool pending = false;
+  unsigned long long rechecked = 0;
+
+  ~culsh_bt() {
+    if (pending && ev1) (void)hipEventSynchronize(ev1);
+    drop_graph();
+    if (hpack) (void)hipHostFree(hpack);
+    if (hres) (void)hipHostFree(hres);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+bool same_schedule(const culsh_schedule& a, const culsh_schedule& b) {
+  return a.probe_queries == b.probe_queries && a.probe_tables == b.probe_tables &&
+         a.tables_per_item == b.tables_per_item && a.rows_in_flight == b.rows_in_flight &&
+         a.wgs_per_cu == b.wgs_per_cu;
+}
+
+int check_schedule(const culsh_schedule& s) {
+  const bool tile = (s.probe_queries == 32 && s.probe_tables == 1) ||
+                    (s.probe_queries == 16 && s.probe_tables == 2) ||
+                    (s.probe_queries == 8 && s.probe_tables == 4);
+  if (!tile) return fail(CULSH_E_ARG, "probe tile must be 32x1, 16x2 or 8x4");
+  if (s.tables_per_item < 1) return fail(CULSH_E_ARG, "tables_per_item must be >= 1");
+  if (s.rows_in_flight != 2 && s.rows_in_flight != 4 && s.rows_in_flight != 8)
+    return fail(CULSH_E_ARG, "rows_in_flight must be 2, 4 or 8");
+  if (s.wgs_per_cu < 1 || s.wgs_per_cu > 3) return fail(CULSH_E_ARG, "wgs_per_cu must be 1..3");
+  return CULSH_OK;
+}
+
+typedef void (*BatchKernel)(BatchArgs);
+
+template <int MI, int OCC>
+BatchKernel batch_kernel_r(int R) {
+  if (R == 2) return k_batch<MI, 2, OCC>;
+  if (R == 8) return k_batch<MI, 8, OCC>;
+  return k_batch<MI, 4, OCC>;
+}
+
+template <int OCC>
+BatchKernel batch_kernel_o(int M, int R) {
+  if (M <= 8) return batch_kernel_r<1, OCC>(R);
+  if (M <= 16) return batch_kernel_r<2, OCC>(R);
+  if (M <= 32) return batch_kernel_r<4, OCC>(R);
+  return batch_kernel_r<8, OCC>(R);
+}
+
+// Three workgroups per CU need the tighter register bound.
+BatchKernel batch_kernel(int M, int R, int wgs_per_cu) {
+  return wgs_per_cu >= 3 ? batch_kernel_o<3>(M, R) : batch_kernel_o<2>(M, R);
+}
+
+BatchArgs batch_args(culsh_bt* bt) {
+  culsh_ht* ht = bt->ht;
+  BatchArgs a;
+  a.hdr = reinterpret_cast<BatchHdr*>(bt->dpack.p);
+  a.classes = reinterpret_cast<const BatchClass*>(bt->dpack.p + bt->off_cls);
+  a.bq = bt->dbq.p;
+  a.blockdone = bt->dblk.p;
+  a.qb = bt->dqb.p;
+  a.cnu_qt = bt->dcnu_qt.p;
+  a.cnu_q = bt->dcnu_q.p;
+  a.cu_q = bt->dcu_q.p;
+  a.ovl = bt->dovl.p;
+  a.X = ht->dX;
+  a.Ad = ht->dAd.p;
+  a.b64 = ht->db64.p;
+  a.seeds = ht->dseeds.p;
+  a.dir = ht->dir.p;
+  a.recs = ht->recs.p;
+  a.bcodes = ht->bcodes.p;
+  a.sids = ht->sids.p;
+  a.W = (double)ht->W;
+  a.D = ht->D;
+  a.L = ht->L;
+  a.M = ht->M;
+  a.Jp = ht->Jp;
+  a.tb = ht->tb;
+  a.shift = ht->shift;
+  return a;
+}
+
+// Grows the plan's buffers to hold a batch of Qtot queries, nreq requests,
+// nblk probe blocks and Ptot probe slots per table; drops the graph when
+// anything moved.
+int bt_reserve(culsh_bt* bt, int Qtot, int nreq, int nblk, long long Ptot) {
+  const int L = bt->ht->L;
+  bool moved = false;
+  if (nreq > bt->Rcap || !bt->hpack) {
+    const int R = std::max(nreq, std::max(64, 2 * bt->Rcap));
+    const size_t off_cls = (sizeof(BatchHdr) + 63) & ~size_t(63);
+    const size_t off_req = (off_cls + R * sizeof(BatchClass) + 63) & ~size_t(63);
+    const size_t packed = off_req + R * sizeof(BatchReq);
+    TRY(bt->dpack.exact(packed));
+    if (bt->hpack) HIP_TRY(hipHostFree(bt->hpack));
+    bt->hpack = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&bt->hpack, packed, hipHostMallocDefault));
+    bt->Rcap = R;
+    bt->off_cls = off_cls;
+    bt->off_req = off_req;
+    bt->packed = packed;
+    moved = true;
+  }
+  if (Qtot > bt->Qcap) {
+    const int Qc = std::max(Qtot, std::min(2 * bt->Qcap, (1 << 30)));
+    TRY(bt->dbq.exact(Qc));
+    TRY(bt->dcnu_qt.exact((size_t)Qc * L));
+    TRY(bt->dcnu_q.exact(Qc));
+    TRY(bt->dcu_q.exact(Qc));
+    TRY(bt->dovl.exact(Qc));
+    bt->Qcap = Qc;
+    moved = true;
+  }
+  if (nblk > bt->Bcap) {
+    const int Bc = std::max(nblk, 2 * bt->Bcap);
+    TRY(bt->dblk.exact(Bc));
+    bt->Bcap = Bc;
+    moved = true;
+  }
+  if (Ptot > bt->Pcap) {
+    const long long Pc = std::max(Ptot, 2 * bt->Pcap);
+    if ((long long)L * Pc >= (1ll << 31))
+      return fail(CULSH_E_LIMIT, "L * (sum of Q*T over the batch) exceeds 2^31 - 1");
+    TRY(bt->dqb.exact((size_t)L * Pc));
+    bt->Pcap = Pc;
+    moved = true;
+  }
+  if (!bt->dscr.p) {
+    TRY(bt->dscr.exact((size_t)culsh_bt::kOvGrid *
+                       ((1ull << culsh_bt::kHbitsMax) + 2ull * culsh_bt::kUcapMax)));
+    moved = true;
+  }
+  if (moved) bt->drop_graph();
+  return CULSH_OK;
+}
+
+// Captures: descriptor upload, expansion, persistent kernel, overflow
+// kernel, header download.  Every pointer is a plan buffer, so the graph
+// serves any batch within the capacities.
+int bt_capture(culsh_bt* bt) {
+  culsh_ht* ht = bt->ht;
+  const culsh_schedule& s = bt->sched;
+  const BatchArgs a = batch_args(bt);
+  const BatchKernel kb = batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu);
+  const int grid = bt->ncu * s.wgs_per_cu;
+  const int egrid = (int)grid_for(std::max(bt->Qcap, bt->Bcap));
+  bt->drop_graph();
+  HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));
+  const hipError_t m1 =
+      hipMemcpyAsync(bt->dpack.p, bt->hpack, bt->packed, hipMemcpyHostToDevice, bt->cap);
+  hipLaunchKernelGGL(k_batch_expand, dim3(egrid), dim3(256), 0, bt->cap, a.hdr,
+                     reinterpret_cast<const BatchReq*>(bt->dpack.p + bt->off_req), a.classes,
+                     bt->dbq.p, bt->dblk.p, bt->Bcap);
+  hipLaunchKernelGGL(kb, dim3(grid), dim3(256), 0, bt->cap, a);
+  hipLaunchKernelGGL(k_batch_overflow, dim3(culsh_bt::kOvGrid), dim3(256), 0, bt->cap, a,
+                     bt->dscr.p, culsh_bt::kHbitsMax, culsh_bt::kUcapMax);
+  const hipError_t m2 =
+      hipMemcpyAsync(bt->hres, bt->dpack.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, bt->cap);
+  hipError_t le = hipGetLastError();
+  if (le == hipSuccess) le = m1 != hipSuccess ? m1 : m2;
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(bt->cap, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(CULSH_E_DEVICE, "batch graph capture failed: %s",
+                hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  bt->graph = g;
+  HIP_TRY(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
+  bt->graph_sched = s;
+  return CULSH_OK;
+}
+
+int bt_wait(culsh_bt* bt) {
+  if (!bt->pending) return CULSH_OK;
+  HIP_TRY(hipEventSynchronize(bt->ev1));
+  bt->pending = false;
+  const BatchHdr& r = *bt->hres;
+  bt->rechecked = r.n_recheck;
+  if (r.err & 2) return fail(CULSH_E_DEVICE, "batch: a candidate item timed out waiting for its probes");
+  if (r.err & 1)
+    return fail(CULSH_E_LIMIT,
+                "batch: a query has more than %u non-unique candidates or more than %d "
+                "candidates within the FP32 error bound of its K-th distance",
+                culsh_bt::kUcapMax, kScapG);
+  return CULSH_OK;
+}
+
+int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ, int* d_ids,
+              float* d_dists, hipStream_t stream) {
+  culsh_ht* ht = bt->ht;
+  if (!ht->indexed || !ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
+  if (nreq <= 0 || !reqs) return fail(CULSH_E_ARG, "empty batch");
+  if (!dQ || !d_ids || !d_dists) return fail(CULSH_E_ARG, "null device pointer");
+  TRY(bt_wait(bt));
+  const culsh_schedule& s = bt->sched;
+  // Batch order: T descending, then K descending, then request order.
+  std::vector<int> ord;
+  ord.reserve(nreq);
+  long long Qsum = 0;
+  for (int r = 0; r < nreq; ++r) {
+    const culsh_request& q = reqs[r];
+    if (q.Q < 0 || q.K <= 0 || q.T <= 0 || q.q_row < 0 || q.out_offset < 0)
+      return fail(CULSH_E_ARG, "request %d: Q >= 0, K > 0, T > 0 and offsets >= 0 required", r);
+    if (q.T > 64) return fail(CULSH_E_LIMIT, "request %d: T = %d exceeds 64 in a batch", r, q.T);
+    if (q.Q > 0) ord.push_back(r);
+    Qsum += q.Q;
+  }
+  if (Qsum >= (1ll << 30)) return fail(CULSH_E_LIMIT, "a batch holds 2^30 or more queries");
+  if (ord.empty()) return CULSH_OK;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    if (reqs[x].T != reqs[y].T) return reqs[x].T > reqs[y].T;
+    return reqs[x].K > reqs[y].K;
+  });
+  const int n = (int)ord.size();
+  // Tables per probe item: at least the tile's table count.
+  const int tgroup = std::max(s.tables_per_item, s.probe_tables);
+  const int ntg = (ht->L + tgroup - 1) / tgroup;
+  std::vector<BatchReq> vr(n);
+  std::vector<BatchClass> vc;
+  int b = 0, blk = 0, item = 0;
+  long long p = 0;
+  for (int j = 0; j < n; ++j) {
+    const culsh_request& q = reqs[ord[j]];
+    if (vc.empty() || vc.back().T != q.T) {
+      if (!vc.empty()) {
+        const int nb = (vc.back().n + s.probe_queries - 1) / s.probe_queries;
+        blk += nb;
+        item += nb * ntg;
+      }
+      BatchClass c;
+      c.b0 = b;
+      c.n = 0;
+      c.T = q.T;
+      c.blk0 = blk;
+      c.item0 = item;
+      c.pad = 0;
+      vc.push_back(c);
+    }
+    vc.back().n += q.Q;
+    BatchReq& br = vr[j];
+    br.qrow0 = q.q_row;
+    br.out0 = q.out_offset;
+    br.p0 = p;
+    br.b0 = b;
+    br.Q = q.Q;
+    br.K = q.K;
+    br.T = q.T;
+    b += q.Q;
+    p += (long long)q.Q * q.T;
+  }
+  {
+    const int nb = (vc.back().n + s.probe_queries - 1) / s.probe_queries;
+    blk += nb;
+    item += nb * ntg;
+  }
+  TRY(bt_reserve(bt, b, n, blk, p));
+  if (!bt->exec || !same_schedule(bt->graph_sched, s)) TRY(bt_capture(bt));
+  BatchHdr h;
+  std::memset(&h, 0, sizeof(h));
+  h.Qm = dQ;
+  h.out_ids = d_ids;
+  h.out_d = d_dists;
+  h.Ptot = bt->Pcap;
+  h.nreq = n;
+  h.nclass = (int)vc.size();
+  h.Qtot = b;
+  h.nprobe = item;
+  h.qn = s.probe_queries;
+  h.tn = s.probe_tables;
+  h.tgroup = tgroup;
+  h.ntg = ntg;
+  std::memcpy(bt->hpack, &h, sizeof(h));
+  std::memcpy(bt->hpack + bt->off_cls, vc.data(), vc.size() * sizeof(BatchClass));
+  std::memcpy(bt->hpack + bt->off_req, vr.data(), vr.size() * sizeof(BatchReq));
+  HIP_TRY(hipEventRecord(bt->ev0, stream));
+  HIP_TRY(hipGraphLaunch(bt->exec, stream));
+  HIP_TRY(hipEventRecord(bt->ev1, stream));
+  bt->pending = true;
+  return CULSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int culsh_bt_create(culsh_ht* ht, culsh_bt** out) {
+  if (!out) return fail(CULSH_E_ARG, "null output pointer");
+  if (!ht) return fail(CULSH_E_ARG, "null HashTables");
+  auto* bt = new culsh_bt;
+  bt->ht = ht;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&bt->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->cap, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&bt->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&bt->ev1);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&bt->hres, sizeof(BatchHdr), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete bt;
+    return fail(CULSH_E_DEVICE, "batch setup: %s", hipGetErrorString(e));
+  }
+  *out = bt;
+  return CULSH_OK;
+}
+
+void culsh_bt_destroy(culsh_bt* bt) { delete bt; }
+
+int culsh_bt_set_schedule(culsh_bt* bt, const culsh_schedule* s) {
+  if (!bt || !s) return fail(CULSH_E_ARG, "null argument");
+  TRY(check_schedule(*s));
+  TRY(bt_wait(bt));
+  bt->sched = *s;
+  return CULSH_OK;
+}
+
+int culsh_bt_get_schedule(culsh_bt* bt, culsh_schedule* s) {
+  if (!bt || !s) return fail(CULSH_E_ARG, "null argument");
+  *s = bt->sched;
+  return CULSH_OK;
+}
+
+int culsh_bt_search_device(culsh_bt* bt, const culsh_request* reqs, int nreq,
+                           const float* d_queries, int* d_ids, float* d_dists, void* stream) {
+  if (!bt) return fail(CULSH_E_ARG, "null batch");
+  return bt_launch(bt, reqs, nreq, d_queries, d_ids, d_dists, (hipStream_t)stream);
+}
+
+int culsh_bt_wait(culsh_bt* bt) {
+  if (!bt) return fail(CULSH_E_ARG, "null batch");
+  return bt_wait(bt);
+}
+
+int culsh_bt_last_ms(culsh_bt* bt, float* ms) {
+  if (!bt || !ms) return fail(CULSH_E_ARG, "null argument");
+  TRY(bt_wait(bt));
+  HIP_TRY(hipEventElapsedTime(ms, bt->ev0, bt->ev1));
+  return CULSH_OK;
+}
+
+unsigned long long culsh_bt_rechecked(culsh_bt* bt) { return bt ? bt->rechecked : 0ull; }
+
+int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
+                      int* d_ids, float* d_dists, int reps, float* default_ms, float* tuned_ms) {
+  if (!bt) return fail(CULSH_E_ARG, "null batch");
+  reps = std::max(1, reps);
+  auto timed = [&](const culsh_schedule& s, float* best) -> int {
+    TRY(culsh_bt_set_schedule(bt, &s));
+    *best = INFINITY;
+    for (int r = 0; r < reps + 1; ++r) {  // the first run also captures the graph
+      TRY(bt_launch(bt, reqs, nreq, d_queries, d_ids, d_dists, bt->cap));
+      float ms = 0.f;
+      TRY(culsh_bt_last_ms(bt, &ms));
+      if (r > 0) *best = std::min(*best, ms);
+    }
+    return CULSH_OK;
+  };
+  const culsh_schedule def{32, 1, 1, 4, 2};
+  culsh_schedule best = def;
+  float tbest = 0.f;
+  TRY(timed(def, &tbest));
+  if (default_ms) *default_ms = tbest;
+  const int tiles[3][2] = {{32, 1}, {16, 2}, {8, 4}};
+  const int tpis[4] = {1, 2, 4, 8};
+  const int rows[3] = {2, 4, 8};
+  const int wgs[3] = {1, 2, 3};
+  auto consider = [&](const culsh_schedule& s) -> int {
+    if (same_schedule(s, best)) return CULSH_OK;
+    float t = 0.f;
+    TRY(timed(s, &t));
+    if (t < 0.99f * tbest) {
+      tbest = t;
+      best = s;
+    }
+    return CULSH_OK;
+  };
+  for (auto& tl : tiles) {
+    culsh_schedule s = best;
+    s.probe_queries = tl[0];
+    s.probe_tables = tl[1];
+    s.tables_per_item = std::max(s.tables_per_item, tl[1]);
+    TRY(consider(s));
+  }
+  for (int v : tpis) {
+    culsh_schedule s = best;
+    if (v < s.probe_tables) continue;
+    s.tables_per_item = v;
+    TRY(consider(s));
+  }
+  for (int v : rows) {
+    culsh_schedule s = best;
+    s.rows_in_flight = v;
+    TRY(consider(s));
+  }
+  for (int v : wgs) {
+    culsh_schedule s = best;
+    s.wgs_per_cu = v;
+    TRY(consider(s));
+  }
+  TRY(culsh_bt_set_schedule(bt, &best));
+  if (tuned_ms) *tuned_ms = tbest;
+  return CULSH_OK;
+}
+
+}  // extern "C"

@@ -57,28 +57,31 @@ __device__ inline double dist64(const float* __restrict__ q, const float* __rest
 }
 
 // Direct FP32 squared L2 of the U candidate rows ids[0..U) against q
-// (cuLSH_Kernels.cu:319-346): 16 lanes per row, four rows per 16-lane group
-// with their loads issued together.  Writes d32[u].
+// (cuLSH_Kernels.cu:319-346): 16 lanes per row, R rows per 16-lane group
+// with their loads issued together (the candidate chunk).  Writes d32[u].
+template <int R = 4>
 __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __restrict__ q,
                                     int D, const uint32_t* ids, int U, float* d32) {
   const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
   if ((D & 3) == 0) {
     const int D4 = D >> 2;
     const float4* q4 = reinterpret_cast<const float4*>(q);
-    for (int u0 = g * 4; u0 < U; u0 += 64) {
-      const float4* xr[4];
+    for (int u0 = g * R; u0 < U; u0 += 16 * R) {
+      const float4* xr[R];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
+      for (int c = 0; c < R; ++c)
         xr[c] = reinterpret_cast<const float4*>(X + (long long)ids[min(u0 + c, U - 1)] * D);
-      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+      float acc[R];
+#pragma unroll
+      for (int c = 0; c < R; ++c) acc[c] = 0.f;
 #pragma unroll 2
       for (int f = lig; f < D4; f += 16) {
-        float4 a[4];
+        float4 a[R];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a[c] = xr[c][f];
+        for (int c = 0; c < R; ++c) a[c] = xr[c][f];
         const float4 b = q4[f];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < R; ++c) {
           const float d0 = b.x - a[c].x, d1 = b.y - a[c].y, d2 = b.z - a[c].z,
                       d3 = b.w - a[c].w;
           acc[c] = fmaf(d0, d0, acc[c]);
@@ -88,13 +91,13 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
         }
       }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < R; ++c) {
 #pragma unroll
         for (int off = 8; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 16);
       }
       if (lig == 0) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < R; ++c)
           if (u0 + c < U) d32[u0 + c] = acc[c];
       }
     }
@@ -113,13 +116,14 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
   }
 }
 
-// Calls ins(id) for every member of every bucket query q matched
+// Calls ins(id) for every member of every bucket a query matched
 // (cuLSH_SearchTables.cu:321-340): a wave takes 64 (table, probe) entries,
 // prefix-sums their bucket sizes and spreads the member ids over its lanes.
-// All threads of the workgroup must call it.
+// qbq[t * tstride + p] is the query's probe p of table t.  All threads of
+// the workgroup must call it.
 template <class Ins>
-__device__ inline void for_each_member(int q, int Q, int L, int T, const int* __restrict__ qb,
-                                       const BucketRec* __restrict__ recs,
+__device__ inline void for_each_member(const int* __restrict__ qbq, long long tstride, int L,
+                                       int T, const BucketRec* __restrict__ recs,
                                        const uint32_t* __restrict__ sids, Ins ins) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int LT = L * T;
@@ -128,7 +132,7 @@ __device__ inline void for_each_member(int q, int Q, int L, int T, const int* __
     uint32_t st = 0, sz = 0;
     if (e < LT) {
       const int t = e / T, p = e - t * T;
-      const int k = qb[(long long)t * Q * T + (long long)q * T + p];
+      const int k = qbq[(long long)t * tstride + p];
       if (k >= 0) {
         st = recs[k].start;
         sz = recs[k].size;
@@ -172,14 +176,13 @@ __device__ inline unsigned long long query_cnu(int q, int L,
 // K-th smallest dapx (radix select over the float bits; distances are >= 0),
 // re-evaluates in FP64 (the oracle's order) every candidate whose dapx can
 // still be at or below the true K-th distance, and applies the reference's
-// tie rule to the FP64 values.  Writes K outputs.  Returns false
-// (uniformly) when more than scap candidates need the FP64 pass.  All
-// threads of the workgroup must call it.
-__device__ inline bool finish_topk(int q, const float* __restrict__ X, const float* y, int D,
-                                   int K, int U, const float* dapx, const uint32_t* ids,
-                                   double rel, double abs_err, double* sd, uint32_t* sid,
-                                   unsigned scap, int* __restrict__ out_ids,
-                                   float* __restrict__ out_d,
+// tie rule to the FP64 values.  Writes the K outputs oid[0..K), od[0..K).
+// Returns false (uniformly) when more than scap candidates need the FP64
+// pass.  All threads of the workgroup must call it.
+__device__ inline bool finish_topk(const float* __restrict__ X, const float* y, int D, int K,
+                                   int U, const float* dapx, const uint32_t* ids, double rel,
+                                   double abs_err, double* sd, uint32_t* sid, unsigned scap,
+                                   int* __restrict__ oid, float* __restrict__ od,
                                    unsigned long long* __restrict__ n_recheck) {
   __shared__ uint32_t s_hist[256];
   __shared__ unsigned s_scount, s_ovf, s_prefix, s_rem, s_misc[4];
@@ -327,8 +330,8 @@ __device__ inline bool finish_topk(int q, const float* __restrict__ X, const flo
       id = (int)sid[src];
       dd = (float)sd[src];
     }
-    out_ids[(long long)q * K + k] = id;
-    out_d[(long long)q * K + k] = dd;
+    oid[k] = id;
+    od[k] = dd;
   }
   __syncthreads();
   return true;
@@ -347,24 +350,53 @@ struct CandBufs {
   float* ys;        // query vector staging (kYcap floats, may alias hash), or null
 };
 
-// Plain form, query q.  Returns false (uniformly) when the query does not
+// Where one query's inputs and outputs live.  `slot` indexes its C_nu row
+// (cnu_qt[slot * L + t]) and its cnu_q / cu_q entries.
+struct QueryIO {
+  const float* y;       // the query vector (D floats)
+  const int* qbq;       // its probe 0 of table 0; probe p of table t at t * tstride + p
+  long long tstride;
+  int T, K;
+  int* oid;             // its K result ids and distances
+  float* od;
+  long long slot;
+};
+
+__device__ inline QueryIO search_io(int q, const float* Qm, int Q, int D, int T, int K,
+                                    const int* qb, int* out_ids, float* out_d) {
+  QueryIO io;
+  io.y = Qm + (long long)q * D;
+  io.qbq = qb + (long long)q * T;
+  io.tstride = (long long)Q * T;
+  io.T = T;
+  io.K = K;
+  io.oid = out_ids + (long long)q * K;
+  io.od = out_d + (long long)q * K;
+  io.slot = q;
+  return io;
+}
+
+// Plain form, one query.  Returns false (uniformly) when the query does not
 // fit the buffers; its outputs are then not written.
-__device__ inline bool query_candidates(
-    int q, const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T,
-    int K, const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
-    int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
-    unsigned long long* __restrict__ cu_q, unsigned long long* __restrict__ n_recheck,
-    const CandBufs& B) {
+template <int R = 4>
+__device__ inline bool query_candidates(const QueryIO& io, const float* __restrict__ X, int D,
+                                        int L, const BucketRec* __restrict__ recs,
+                                        const uint32_t* __restrict__ sids,
+                                        const unsigned long long* __restrict__ cnu_qt,
+                                        unsigned long long* __restrict__ cnu_q,
+                                        unsigned long long* __restrict__ cu_q,
+                                        unsigned long long* __restrict__ n_recheck,
+                                        const CandBufs& B) {
   __shared__ unsigned long long s_cnu;
   __shared__ unsigned s_count, s_ovf;
   const int tid = threadIdx.x;
-  const float* y = Qm + (long long)q * D;
+  const float* y = io.y;
+  const int K = io.K;
   const uint32_t hslots = 1u << B.hbits, hmask = hslots - 1;
 
   __syncthreads();  // a previous query's readers of the shared counters are done
   if (tid == 0) {
-    s_cnu = query_cnu(q, L, cnu_qt);
+    s_cnu = query_cnu((int)io.slot, L, cnu_qt);
     s_count = 0;
     s_ovf = 0;
   }
@@ -372,7 +404,7 @@ __device__ inline bool query_candidates(
   __syncthreads();
   const unsigned long long cnu = s_cnu;
   if (cnu > 0) {
-    for_each_member(q, Q, L, T, qb, recs, sids, [&](uint32_t id) {
+    for_each_member(io.qbq, io.tstride, L, io.T, recs, sids, [&](uint32_t id) {
       uint32_t h = (id * 0x9E3779B1u) >> (32 - B.hbits);
       for (uint32_t probe = 0;; ++probe) {
         if (probe == hslots) {
@@ -403,17 +435,17 @@ __device__ inline bool query_candidates(
     for (int d = tid; d < D; d += 256) B.ys[d] = y[d];
   __syncthreads();
   if (y_staged)
-    l2_fp32_rows(X, B.ys, D, B.ulist, U, B.d32);
+    l2_fp32_rows<R>(X, B.ys, D, B.ulist, U, B.d32);
   else
-    l2_fp32_rows(X, y, D, B.ulist, U, B.d32);
+    l2_fp32_rows<R>(X, y, D, B.ulist, U, B.d32);
   // Direct FP32: |d32 - d| <= (D + 3) u d; margin 2 (Higham ch. 3).
   const double rel = 2.0 * (double)(D + 8) * kU32d;
-  if (!finish_topk(q, X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd, B.sid,
-                   B.scap, out_ids, out_d, n_recheck))
+  if (!finish_topk(X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd, B.sid,
+                   B.scap, io.oid, io.od, n_recheck))
     return false;
   if (tid == 0) {
-    cnu_q[q] = cnu;
-    cu_q[q] = (unsigned long long)U;
+    cnu_q[io.slot] = cnu;
+    cu_q[io.slot] = (unsigned long long)U;
   }
   return true;
 }
@@ -452,8 +484,8 @@ __global__ __launch_bounds__(256) void k_candidates(
     if (pos >= Q) return;
     q = (int)order[pos];
   }
-  if (!query_candidates(q, X, Qm, Q, D, L, T, K, qb, recs, sids, cnu_qt, out_ids, out_d, cnu_q,
-                        cu_q, n_recheck, B)) {
+  if (!query_candidates(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
+                        cnu_qt, cnu_q, cu_q, n_recheck, B)) {
     if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1)] = q;
   }
 }
@@ -494,8 +526,8 @@ __global__ __launch_bounds__(256) void k_candidates_global(
   B.scap = kScapG;
   B.ys = s_y;
   for (int i = blockIdx.x; i < n_list; i += gridDim.x) {
-    if (!query_candidates(list[i], X, Qm, Q, D, L, T, K, qb, recs, sids, cnu_qt, out_ids, out_d,
-                          cnu_q, cu_q, n_recheck, B)) {
+    if (!query_candidates(search_io(list[i], Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs,
+                          sids, cnu_qt, cnu_q, cu_q, n_recheck, B)) {
       if (threadIdx.x == 0) atomicOr(err, 1);
     }
   }
@@ -626,7 +658,7 @@ __global__ __launch_bounds__(256) void k_group_candidates(
     // Union of the group's candidates; hash slot masks record membership.
     for (int g = 0; g < gs; ++g) {
       const int q = s_qid[g];
-      for_each_member(q, Q, L, T, qb, recs, sids, [&](uint32_t id) {
+      for_each_member(qb + (long long)q * T, (long long)Q * T, L, T, recs, sids, [&](uint32_t id) {
         uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
         for (uint32_t probe = 0;; ++probe) {
           if (probe == hslots) {
@@ -725,9 +757,10 @@ __global__ __launch_bounds__(256) void k_group_candidates(
       const int q = s_qid[g];
       const double qb2 = (double)s_qnorm[g] + (double)xmax;
       const double abs_err = 2.0 * (double)(D + 4) * kU32d * qb2 * qb2 * (1.0 + 1e-6);
-      const bool ok = finish_topk(q, X, Qm + (long long)q * D, D, K, (int)cu[g], qd + s_qoff[g],
-                                  qi + s_qoff[g], 0.0, abs_err, s_sd, s_sid, kScap, out_ids,
-                                  out_d, n_recheck);
+      const bool ok = finish_topk(X, Qm + (long long)q * D, D, K, (int)cu[g], qd + s_qoff[g],
+                                  qi + s_qoff[g], 0.0, abs_err, s_sd, s_sid, kScap,
+                                  out_ids + (long long)q * K, out_d + (long long)q * K,
+                                  n_recheck);
       if (tid == 0) {
         if (ok) {
           cnu_q[q] = s_cnu[g];

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/culsh_c.h"
+#include "batch_kernels.hpp"
 #include "common.hpp"
 #include "index_kernels.hpp"
 #include "philox.hpp"
@@ -1269,3 +1270,7 @@ const unsigned long long* culsh_st_query_cu(culsh_st* st) {
 unsigned long long culsh_st_rechecked(culsh_st* st) { return st ? st->rechecked : 0; }
 
 }  // extern "C"
+
+// Batching form (culsh_bt_*): shares this translation unit's helpers and
+// index state.
+#include "batch_host.hpp"

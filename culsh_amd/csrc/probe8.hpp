@@ -65,39 +65,34 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
   return hs;
 }
 
+// One (query, table) pair per octet: FP64 projection of y (the oracle's
+// order), deltas, delta order, the multiprobe queue, keys, lookup and verify.
 // J = ceil(T / 8) queue slots per lane (slot s: lane s%8, register s/8);
 // MI >= ceil(M / 8) dimensions per lane (lane l owns l + 8i), a compile-time
 // bound so that the projection loop is branch-free.  Ad is the [D][Jp] f32
 // projection matrix (column t*M + m) with at least 8*MI floats of padding
 // after its last row, read through L1 (lanes past M read padding or the next
 // table's columns and discard them); the FP64 product of f32 operands is
-// exact, so widening on read is the oracle's arithmetic.  Needs
-// probe8_lds_bytes(D, M) bytes of dynamic LDS.
+// exact, so widening on read is the oracle's arithmetic.  orb is the octet's
+// probe8_octet_bytes(M) of LDS.  Writes qbp[p] (bucket of probe p, or -1)
+// for p < T and *cnu (summed sizes of the matched buckets) when active.
+// Every lane of the octet calls it; T is uniform over the wave.
 template <int J, int MI>
-__global__ __launch_bounds__(256) void k_query_probe8(
-    const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
-    const float* __restrict__ Ad, int Jp, const double* __restrict__ b64, double W,
-    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
-    const BucketRec* __restrict__ recs, const int16_t* __restrict__ bcodes,
-    int* __restrict__ qb, unsigned long long* __restrict__ cnu_qt) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int tid = threadIdx.x, lane8 = tid & 7, oct = tid >> 3;
-  const int t = blockIdx.x % L;
-  const int q0 = (blockIdx.x / L) * 32;
-  const int q = q0 + oct;
-  const bool active = q < Q;
-  float* Ys = reinterpret_cast<float*>(smem);
-  unsigned char* orb = smem + ((32 * D * 4 + 15) & ~15) + oct * probe8_octet_bytes(M);
+__device__ inline void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
+                                   const float* __restrict__ Ad, int Jp,
+                                   const double* __restrict__ b64, double W,
+                                   const uint64_t* __restrict__ seeds, int tb, int shift,
+                                   const uint64_t* __restrict__ dir,
+                                   const BucketRec* __restrict__ recs,
+                                   const int16_t* __restrict__ bcodes, unsigned char* orb,
+                                   int* __restrict__ qbp, unsigned long long* __restrict__ cnu) {
+  const int lane8 = threadIdx.x & 7;
   double* ds = reinterpret_cast<double*>(orb);
   signed char* perm = reinterpret_cast<signed char*>(ds + 2 * M);
   unsigned char* rank = reinterpret_cast<unsigned char*>(perm + 64);
   int* base = reinterpret_cast<int*>(rank + 128);
-
-  // Stage the block's query vectors (coalesced).
-  const int nq = min(32, Q - q0);
-  for (int e = tid; e < nq * D; e += 256) Ys[e] = Qm[(long long)q0 * D + e];
   for (int r = lane8; r < 128; r += 8) rank[r] = 255;
-  __syncthreads();
+  wave_lds_sync();
 
   // FP64 projection, the oracle's order; lane l owns dimensions l + 8i.
   bool valid[MI];
@@ -107,7 +102,6 @@ __global__ __launch_bounds__(256) void k_query_probe8(
     valid[i] = lane8 + 8 * i < M;
     acc[i] = 0.0;
   }
-  const float* y = Ys + (active ? oct : 0) * D;
   const float* acol = Ad + t * M + lane8;
 #pragma unroll 8
   for (int d = 0; d < D; ++d) {
@@ -269,7 +263,6 @@ __global__ __launch_bounds__(256) void k_query_probe8(
   // Keys, lookup, verify: lane l handles probes l + 8r.  All of a lane's
   // directory loads are issued before any is resolved.
   unsigned long long sizesum = 0;
-  const long long qbBase = (long long)t * Q * T + (long long)q * T;
   uint64_t key[J], ent[J];
 #pragma unroll
   for (int r = 0; r < J; ++r) {
@@ -289,12 +282,40 @@ __global__ __launch_bounds__(256) void k_query_probe8(
     if (active && p < T) {
       long long k = lookup_resolve(key[r], ent[r], dir, shift, recs);
       if (k >= 0 && !probe_matches(bcodes + k * M, base, perm, rank, M, pm[r])) k = -1;
-      qb[qbBase + p] = (int)k;
+      qbp[p] = (int)k;
       if (k >= 0) sizesum += recs[k].size;
     }
   }
   sizesum = octet_sum_u64(sizesum);
-  if (active && lane8 == 0) cnu_qt[(long long)q * L + t] = sizesum;
+  if (active && lane8 == 0) *cnu = sizesum;
+}
+
+// The search's octet kernel: 32 queries of one table per workgroup, query
+// vectors staged in LDS.  qb[t][q*T + p], cnu_qt[q*L + t].  Needs
+// probe8_lds_bytes(D, M) bytes of dynamic LDS.
+template <int J, int MI>
+__global__ __launch_bounds__(256) void k_query_probe8(
+    const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
+    const float* __restrict__ Ad, int Jp, const double* __restrict__ b64, double W,
+    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
+    const BucketRec* __restrict__ recs, const int16_t* __restrict__ bcodes,
+    int* __restrict__ qb, unsigned long long* __restrict__ cnu_qt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, oct = tid >> 3;
+  const int t = blockIdx.x % L;
+  const int q0 = (blockIdx.x / L) * 32;
+  const int q = q0 + oct;
+  const bool active = q < Q;
+  float* Ys = reinterpret_cast<float*>(smem);
+  unsigned char* orb = smem + ((32 * D * 4 + 15) & ~15) + oct * probe8_octet_bytes(M);
+  // Stage the block's query vectors (coalesced).
+  const int nq = min(32, Q - q0);
+  for (int e = tid; e < nq * D; e += 256) Ys[e] = Qm[(long long)q0 * D + e];
+  __syncthreads();
+  const int qa = active ? q : q0;
+  probe8_pair<J, MI>(Ys + (active ? oct : 0) * D, active, t, D, M, T, Ad, Jp, b64, W, seeds, tb,
+                     shift, dir, recs, bcodes, orb, qb + (long long)t * Q * T + (long long)qa * T,
+                     cnu_qt + (long long)qa * L + t);
 }
 
 }  // namespace culsh

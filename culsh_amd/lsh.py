@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FORM_PLAIN, FORM_PRECISION, Times  # noqa: F401
+from ._lib import FORM_PLAIN, FORM_PRECISION, Request, Schedule, Times  # noqa: F401
 
 
 def _f32(a):
@@ -247,3 +247,83 @@ class SearchTables(_Base):
         t = Times()
         self._L.culsh_st_times(self._h, ctypes.byref(t))
         return {f: getattr(t, f) for f, _ in Times._fields_}
+
+
+def _requests(reqs):
+    """[(Q, K, T, q_row, out_offset), ...] -> ctypes array of Request."""
+    arr = (Request * len(reqs))()
+    for i, (Q, K, T, row, off) in enumerate(reqs):
+        arr[i] = Request(int(Q), int(K), int(T), 0, int(row), int(off))
+    return arr
+
+
+def layout_requests(shapes):
+    """Packs requests [(Q, K, T), ...] back to back: returns the request
+    tuples (Q, K, T, q_row, out_offset), the total query rows and the total
+    result elements."""
+    out, row, off = [], 0, 0
+    for Q, K, T in shapes:
+        out.append((Q, K, T, row, off))
+        row += Q
+        off += Q * K
+    return out, row, off
+
+
+class Batch(_Base):
+    """Batching form: independent requests, each with its own Q, K and T,
+    searched against one index by one graph launch (persistent fused kernel).
+    Each request's results equal a SearchTables search of it alone."""
+
+    def __init__(self, hashtables):
+        self._L = _lib.lib()
+        h = ctypes.c_void_p()
+        if self._L.culsh_bt_create(hashtables.handle, ctypes.byref(h)) != 0:
+            raise RuntimeError(_lib.last_error())
+        self._h = h
+        self._ht = hashtables
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.culsh_bt_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def set_schedule(self, probe_queries=32, probe_tables=1, tables_per_item=1,
+                     rows_in_flight=4, wgs_per_cu=2):
+        s = Schedule(probe_queries, probe_tables, tables_per_item, rows_in_flight, wgs_per_cu)
+        return self._ok(self._L.culsh_bt_set_schedule(self._h, ctypes.byref(s)))
+
+    def get_schedule(self):
+        s = Schedule()
+        self._ok(self._L.culsh_bt_get_schedule(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Schedule._fields_}
+
+    def search_device(self, reqs, queries, ids, dists, stream=0, wait=True):
+        """reqs: [(Q, K, T, q_row, out_offset)]; queries: device (rows, D) f32 tensor;
+        ids / dists: flat device i32 / f32 tensors holding every request's [K x Q]."""
+        arr = _requests(reqs)
+        ok = self._ok(self._L.culsh_bt_search_device(self._h, arr, len(reqs), queries.data_ptr(),
+                                                     ids.data_ptr(), dists.data_ptr(),
+                                                     ctypes.c_void_p(stream)))
+        if ok and wait:
+            ok = self.wait()
+        return ok
+
+    def wait(self):
+        return self._ok(self._L.culsh_bt_wait(self._h))
+
+    def last_ms(self):
+        v = ctypes.c_float()
+        return v.value if self._ok(self._L.culsh_bt_last_ms(self._h, ctypes.byref(v))) else None
+
+    def getRechecked(self):
+        return int(self._L.culsh_bt_rechecked(self._h))
+
+    def autotune(self, reqs, queries, ids, dists, reps=2):
+        """Returns (default ms, tuned ms); the tuned schedule stays set."""
+        arr = _requests(reqs)
+        a, b = ctypes.c_float(), ctypes.c_float()
+        if not self._ok(self._L.culsh_bt_autotune(self._h, arr, len(reqs), queries.data_ptr(),
+                                                  ids.data_ptr(), dists.data_ptr(), int(reps),
+                                                  ctypes.byref(a), ctypes.byref(b))):
+            return None
+        return a.value, b.value

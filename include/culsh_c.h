@@ -139,6 +139,54 @@ unsigned long long culsh_st_rechecked(culsh_st* st);
  * (gather + dedup + distances + top-K). */
 int culsh_st_kernel_ms(culsh_st* st, float* probe_ms, float* candidates_ms);
 
+/* ---- Batches of independent requests (batching form) -------------------
+ * No counterpart in the reference, which searches one (K, T) per call
+ * (cuLSH_SearchTables.cu:197-463).  A batch holds requests with their own
+ * Q, K and T against one index; each request's results are exactly those of
+ * culsh_st_reset(ht, K, T) + culsh_st_search_device on its queries alone.
+ * The whole batch runs as one hipGraph launch: a descriptor copy, one
+ * expansion kernel, one persistent fused probe + candidate kernel and one
+ * kernel for queries whose candidates overflow on-chip memory. */
+typedef struct culsh_bt culsh_bt;
+
+typedef struct {
+  int Q, K, T;         /* T <= 64 */
+  int reserved;        /* 0 */
+  long long q_row;     /* first query row in d_queries (row-major [rows][D]) */
+  long long out_offset;/* first element of its [K x Q] results in d_ids / d_dists */
+} culsh_request;
+
+/* Schedule of the persistent kernel (picked by culsh_bt_autotune). */
+typedef struct {
+  int probe_queries;   /* projection tile: queries per pass (32, 16 or 8) */
+  int probe_tables;    /* projection tile: tables per pass (32 / probe_queries) */
+  int tables_per_item; /* probe-group width: tables one probe item covers */
+  int rows_in_flight;  /* candidate chunk: rows per 16-lane group (2, 4, 8) */
+  int wgs_per_cu;      /* persistent workgroups per compute unit (1..3) */
+} culsh_schedule;
+
+int culsh_bt_create(culsh_ht* ht, culsh_bt** out);
+void culsh_bt_destroy(culsh_bt* bt);
+int culsh_bt_set_schedule(culsh_bt* bt, const culsh_schedule* s);
+int culsh_bt_get_schedule(culsh_bt* bt, culsh_schedule* s);
+/* Enqueues the batch on `stream` and returns; d_queries, d_ids and d_dists
+ * are device pointers that stay valid until culsh_bt_wait returns.  Waits
+ * first for this object's previous batch. */
+int culsh_bt_search_device(culsh_bt* bt, const culsh_request* reqs, int nreq,
+                           const float* d_queries, int* d_ids, float* d_dists, void* stream);
+/* Waits for the last batch; CULSH_E_LIMIT if a query exceeded a capacity
+ * (its results are then not written). */
+int culsh_bt_wait(culsh_bt* bt);
+/* Device time (ms) of the last batch, HIP events around the graph launch. */
+int culsh_bt_last_ms(culsh_bt* bt, float* ms);
+/* Candidates re-evaluated in FP64 by the last batch. */
+unsigned long long culsh_bt_rechecked(culsh_bt* bt);
+/* Coordinate search over the schedule on this batch (reps timed runs per
+ * candidate schedule, best kept); leaves the best schedule set.  Reports the
+ * default and tuned batch times (ms). */
+int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
+                      int* d_ids, float* d_dists, int reps, float* default_ms, float* tuned_ms);
+
 #ifdef __cplusplus
 }
 #endif
