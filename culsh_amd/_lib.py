@@ -26,8 +26,8 @@ class Request(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("probe_queries", c_int), ("probe_tables", c_int), ("tables_per_item", c_int),
-                ("rows_in_flight", c_int), ("wgs_per_cu", c_int)]
+    _fields_ = [("probe_queries", c_int), ("probe_tables", c_int), ("queries_per_item", c_int),
+                ("rows_in_flight", c_int), ("wgs_per_cu", c_int), ("fused", c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/culsh_c.h declares.

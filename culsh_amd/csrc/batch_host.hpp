@@ -6,19 +6,18 @@
 
 struct culsh_bt {
   culsh_ht* ht = nullptr;
-  culsh_schedule sched{32, 1, 1, 4, 2};
+  culsh_schedule sched{32, 1, 32, 4, 3, 0};
   hipStream_t cap = nullptr;  // capture stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int ncu = 0;
   // Capacities of the device buffers the graph was captured with.
-  int Qcap = 0, Rcap = 0, Bcap = 0;
+  int Qcap = 0, Rcap = 0;
   long long Pcap = 0;
   size_t off_cls = 0, off_req = 0, packed = 0;
   DevBuf<unsigned char> dpack;  // BatchHdr | BatchClass[Rcap] | BatchReq[Rcap]
   unsigned char* hpack = nullptr;  // pinned host image of dpack
   BatchHdr* hres = nullptr;        // pinned: header after the batch
   DevBuf<BatchQuery> dbq;
-  DevBuf<unsigned> dblk;
   DevBuf<int> dqb, dovl;
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q;
   DevBuf<uint32_t> dscr;
@@ -27,7 +26,7 @@ struct culsh_bt {
   static constexpr unsigned kUcapMax = 1u << 20;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  culsh_schedule graph_sched{0, 0, 0, 0, 0};
+  culsh_schedule graph_sched{0, 0, 0, 0, 0, 0};
   bool pending = false;
   unsigned long long rechecked = 0;
 
@@ -52,8 +51,8 @@ namespace {
 
 bool same_schedule(const culsh_schedule& a, const culsh_schedule& b) {
   return a.probe_queries == b.probe_queries && a.probe_tables == b.probe_tables &&
-         a.tables_per_item == b.tables_per_item && a.rows_in_flight == b.rows_in_flight &&
-         a.wgs_per_cu == b.wgs_per_cu;
+         a.queries_per_item == b.queries_per_item && a.rows_in_flight == b.rows_in_flight &&
+         a.wgs_per_cu == b.wgs_per_cu && a.fused == b.fused;
 }
 
 int check_schedule(const culsh_schedule& s) {
@@ -61,10 +60,14 @@ int check_schedule(const culsh_schedule& s) {
                     (s.probe_queries == 16 && s.probe_tables == 2) ||
                     (s.probe_queries == 8 && s.probe_tables == 4);
   if (!tile) return fail(CULSH_E_ARG, "probe tile must be 32x1, 16x2 or 8x4");
-  if (s.tables_per_item < 1) return fail(CULSH_E_ARG, "tables_per_item must be >= 1");
+  if (s.queries_per_item < s.probe_queries || s.queries_per_item > 256 ||
+      s.queries_per_item % s.probe_queries != 0)
+    return fail(CULSH_E_ARG,
+                "queries_per_item must be a multiple of probe_queries, at most 256");
   if (s.rows_in_flight != 2 && s.rows_in_flight != 4 && s.rows_in_flight != 8)
     return fail(CULSH_E_ARG, "rows_in_flight must be 2, 4 or 8");
   if (s.wgs_per_cu < 1 || s.wgs_per_cu > 3) return fail(CULSH_E_ARG, "wgs_per_cu must be 1..3");
+  if (s.fused != 0 && s.fused != 1) return fail(CULSH_E_ARG, "fused must be 0 or 1");
   return CULSH_OK;
 }
 
@@ -90,13 +93,25 @@ BatchKernel batch_kernel(int M, int R, int wgs_per_cu) {
   return wgs_per_cu >= 3 ? batch_kernel_o<3>(M, R) : batch_kernel_o<2>(M, R);
 }
 
+BatchKernel batch_probe_kernel(int M) {
+  if (M <= 8) return k_batch_probe<1>;
+  if (M <= 16) return k_batch_probe<2>;
+  if (M <= 32) return k_batch_probe<4>;
+  return k_batch_probe<8>;
+}
+
+BatchKernel batch_cand_kernel(int R) {
+  if (R == 2) return k_batch_cand<2>;
+  if (R == 8) return k_batch_cand<8>;
+  return k_batch_cand<4>;
+}
+
 BatchArgs batch_args(culsh_bt* bt) {
   culsh_ht* ht = bt->ht;
   BatchArgs a;
   a.hdr = reinterpret_cast<BatchHdr*>(bt->dpack.p);
   a.classes = reinterpret_cast<const BatchClass*>(bt->dpack.p + bt->off_cls);
   a.bq = bt->dbq.p;
-  a.blockdone = bt->dblk.p;
   a.qb = bt->dqb.p;
   a.cnu_qt = bt->dcnu_qt.p;
   a.cnu_q = bt->dcnu_q.p;
@@ -120,10 +135,9 @@ BatchArgs batch_args(culsh_bt* bt) {
   return a;
 }
 
-// Grows the plan's buffers to hold a batch of Qtot queries, nreq requests,
-// nblk probe blocks and Ptot probe slots per table; drops the graph when
-// anything moved.
-int bt_reserve(culsh_bt* bt, int Qtot, int nreq, int nblk, long long Ptot) {
+// Grows the plan's buffers to hold a batch of Qtot queries, nreq requests
+// and Ptot probe slots per table; drops the graph when anything moved.
+int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
   const int L = bt->ht->L;
   bool moved = false;
   if (nreq > bt->Rcap || !bt->hpack) {
@@ -151,12 +165,6 @@ int bt_reserve(culsh_bt* bt, int Qtot, int nreq, int nblk, long long Ptot) {
     bt->Qcap = Qc;
     moved = true;
   }
-  if (nblk > bt->Bcap) {
-    const int Bc = std::max(nblk, 2 * bt->Bcap);
-    TRY(bt->dblk.exact(Bc));
-    bt->Bcap = Bc;
-    moved = true;
-  }
   if (Ptot > bt->Pcap) {
     const long long Pc = std::max(Ptot, 2 * bt->Pcap);
     if ((long long)L * Pc >= (1ll << 31))
@@ -181,17 +189,23 @@ int bt_capture(culsh_bt* bt) {
   culsh_ht* ht = bt->ht;
   const culsh_schedule& s = bt->sched;
   const BatchArgs a = batch_args(bt);
-  const BatchKernel kb = batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu);
   const int grid = bt->ncu * s.wgs_per_cu;
-  const int egrid = (int)grid_for(std::max(bt->Qcap, bt->Bcap));
+  const int egrid = (int)grid_for(bt->Qcap);
   bt->drop_graph();
   HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));
   const hipError_t m1 =
       hipMemcpyAsync(bt->dpack.p, bt->hpack, bt->packed, hipMemcpyHostToDevice, bt->cap);
   hipLaunchKernelGGL(k_batch_expand, dim3(egrid), dim3(256), 0, bt->cap, a.hdr,
-                     reinterpret_cast<const BatchReq*>(bt->dpack.p + bt->off_req), a.classes,
-                     bt->dbq.p, bt->dblk.p, bt->Bcap);
-  hipLaunchKernelGGL(kb, dim3(grid), dim3(256), 0, bt->cap, a);
+                     reinterpret_cast<const BatchReq*>(bt->dpack.p + bt->off_req), bt->dbq.p);
+  if (s.fused) {
+    hipLaunchKernelGGL(batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu), dim3(grid),
+                       dim3(256), 0, bt->cap, a);
+  } else {
+    hipLaunchKernelGGL(batch_probe_kernel(ht->M), dim3(bt->ncu * 8), dim3(256),
+                       batch_probe_lds(s.probe_queries, ht->D, ht->M), bt->cap, a);
+    hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0, bt->cap,
+                       a);
+  }
   hipLaunchKernelGGL(k_batch_overflow, dim3(culsh_bt::kOvGrid), dim3(256), 0, bt->cap, a,
                      bt->dscr.p, culsh_bt::kHbitsMax, culsh_bt::kUcapMax);
   const hipError_t m2 =
@@ -217,7 +231,6 @@ int bt_wait(culsh_bt* bt) {
   bt->pending = false;
   const BatchHdr& r = *bt->hres;
   bt->rechecked = r.n_recheck;
-  if (r.err & 2) return fail(CULSH_E_DEVICE, "batch: a candidate item timed out waiting for its probes");
   if (r.err & 1)
     return fail(CULSH_E_LIMIT,
                 "batch: a query has more than %u non-unique candidates or more than %d "
@@ -253,27 +266,25 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
     return reqs[x].K > reqs[y].K;
   });
   const int n = (int)ord.size();
-  // Tables per probe item: at least the tile's table count.
-  const int tgroup = std::max(s.tables_per_item, s.probe_tables);
-  const int ntg = (ht->L + tgroup - 1) / tgroup;
+  const int bq = s.queries_per_item;
   std::vector<BatchReq> vr(n);
   std::vector<BatchClass> vc;
-  int b = 0, blk = 0, item = 0;
+  const int ntg = (ht->L + s.probe_tables - 1) / s.probe_tables;
+  int b = 0, item = 0, pitem = 0;
   long long p = 0;
   for (int j = 0; j < n; ++j) {
     const culsh_request& q = reqs[ord[j]];
     if (vc.empty() || vc.back().T != q.T) {
       if (!vc.empty()) {
-        const int nb = (vc.back().n + s.probe_queries - 1) / s.probe_queries;
-        blk += nb;
-        item += nb * ntg;
+        item += (vc.back().n + bq - 1) / bq;
+        pitem += (vc.back().n + s.probe_queries - 1) / s.probe_queries * ntg;
       }
       BatchClass c;
       c.b0 = b;
       c.n = 0;
       c.T = q.T;
-      c.blk0 = blk;
       c.item0 = item;
+      c.pitem0 = pitem;
       c.pad = 0;
       vc.push_back(c);
     }
@@ -289,12 +300,9 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
     b += q.Q;
     p += (long long)q.Q * q.T;
   }
-  {
-    const int nb = (vc.back().n + s.probe_queries - 1) / s.probe_queries;
-    blk += nb;
-    item += nb * ntg;
-  }
-  TRY(bt_reserve(bt, b, n, blk, p));
+  item += (vc.back().n + bq - 1) / bq;
+  pitem += (vc.back().n + s.probe_queries - 1) / s.probe_queries * ntg;
+  TRY(bt_reserve(bt, b, n, p));
   if (!bt->exec || !same_schedule(bt->graph_sched, s)) TRY(bt_capture(bt));
   BatchHdr h;
   std::memset(&h, 0, sizeof(h));
@@ -305,11 +313,11 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
   h.nreq = n;
   h.nclass = (int)vc.size();
   h.Qtot = b;
-  h.nprobe = item;
+  h.nitems = item;
+  h.nprobe = pitem;
   h.qn = s.probe_queries;
   h.tn = s.probe_tables;
-  h.tgroup = tgroup;
-  h.ntg = ntg;
+  h.bq = bq;
   std::memcpy(bt->hpack, &h, sizeof(h));
   std::memcpy(bt->hpack + bt->off_cls, vc.data(), vc.size() * sizeof(BatchClass));
   std::memcpy(bt->hpack + bt->off_req, vr.data(), vr.size() * sizeof(BatchReq));
@@ -395,13 +403,13 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
     }
     return CULSH_OK;
   };
-  const culsh_schedule def{32, 1, 1, 4, 2};
+  const culsh_schedule def{32, 1, 32, 4, 3, 0};
   culsh_schedule best = def;
   float tbest = 0.f;
   TRY(timed(def, &tbest));
   if (default_ms) *default_ms = tbest;
   const int tiles[3][2] = {{32, 1}, {16, 2}, {8, 4}};
-  const int tpis[4] = {1, 2, 4, 8};
+  const int qpis[4] = {8, 16, 32, 64};
   const int rows[3] = {2, 4, 8};
   const int wgs[3] = {1, 2, 3};
   auto consider = [&](const culsh_schedule& s) -> int {
@@ -418,13 +426,13 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
     culsh_schedule s = best;
     s.probe_queries = tl[0];
     s.probe_tables = tl[1];
-    s.tables_per_item = std::max(s.tables_per_item, tl[1]);
+    s.queries_per_item = std::max(tl[0], s.queries_per_item / tl[0] * tl[0]);
     TRY(consider(s));
   }
-  for (int v : tpis) {
+  for (int v : qpis) {
     culsh_schedule s = best;
-    if (v < s.probe_tables) continue;
-    s.tables_per_item = v;
+    if (v < s.probe_queries) continue;
+    s.queries_per_item = v;
     TRY(consider(s));
   }
   for (int v : rows) {
@@ -435,6 +443,11 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
   for (int v : wgs) {
     culsh_schedule s = best;
     s.wgs_per_cu = v;
+    TRY(consider(s));
+  }
+  for (int f = 0; f < 2; ++f) {
+    culsh_schedule s = best;
+    s.fused = f;
     TRY(consider(s));
   }
   TRY(culsh_bt_set_schedule(bt, &best));

@@ -1,18 +1,19 @@
 // Batching form: many independent requests (each with its own Q, K and T)
-// searched by one persistent fused kernel.
+// searched by one graph launch.  Two schedules, picked by the autotuner:
+//   fused  one persistent kernel (k_batch), described below;
+//   split  a grid-stride probe kernel over every request (k_batch_probe, at
+//          the probe's own occupancy) followed by a persistent candidate
+//          kernel (k_batch_cand) with a work queue over the queries.
 //
 // A batch is laid out in "batch order": requests sorted by T (descending),
 // then K (descending), their queries consecutive.  Runs of equal T form
-// classes; a class is cut into probe blocks of `qn` queries.  The work queue
-// holds
-//   probe items     (class, block, table group): qn x tn (query, table) pairs
-//                   per pass, 32 pairs per workgroup (one octet each), over
-//                   `tgroup` tables;
-//   candidate items one per query, in batch order (heaviest T and K first).
-// Workgroups claim items from one atomic ticket counter.  Every probe item
-// has a lower ticket than every candidate item, so a candidate item whose
-// probe block is unfinished waits only on items already claimed by running
-// workgroups; the wait is bounded and reports a timeout instead of hanging.
+// classes; a class is cut into blocks of `bq` queries.  One work item is one
+// block: the workgroup runs the probe stage for the block's queries over all
+// L tables (qn x tn (query, table) pairs per pass, one octet each), then the
+// candidate stage for each of its queries.  A block's probe results are
+// produced and consumed by the same workgroup, so the only synchronisation
+// is the workgroup barrier; workgroups claim blocks from one atomic ticket
+// counter, heaviest class first.
 // Per query, the work is exactly the search's: probe8_pair (cuLSH_Querying.cu
 // :63-225) and query_candidates (cuLSH_SearchTables.cu:249-389,
 // cuLSH_Kernels.cu:301-456), so results are identical to a search of each
@@ -36,8 +37,8 @@ struct BatchReq {
 // A run of batch indices with equal T.
 struct BatchClass {
   int b0, n, T;
-  int blk0;   // first global probe block
-  int item0;  // first probe item
+  int item0;   // first work item of the fused schedule (block of bq queries)
+  int pitem0;  // first probe item of the split schedule
   int pad;
 };
 
@@ -47,20 +48,21 @@ struct BatchHdr {
   int* out_ids;     // caller's results
   float* out_d;
   long long Ptot;   // probe slots per table
-  int nreq, nclass, Qtot, nprobe;
-  int qn, tn, tgroup, ntg;  // schedule: probe tile qn x tn, tables per item, items per block
+  int nreq, nclass, Qtot, nitems;  // nitems: fused work items
+  int nprobe;                      // split-schedule probe items
+  int qn, tn, bq;  // schedule: probe tile qn x tn, queries per fused block
   // Device-written counters (zero in the host copy).
   unsigned next_item;
   int n_overflow;
-  int err;  // bit 0: a query exceeds the overflow path; bit 1: wait timeout
-  int pad;
+  int err;  // bit 0: a query exceeds the overflow path
+  int pad1;
   unsigned long long n_recheck;
 };
 
 // Per-query descriptor, in batch order.
 struct BatchQuery {
   long long qrow, out, poff;
-  int K, T, blk, pad;
+  int K, T;
 };
 
 // Index and plan pointers of the persistent kernel.
@@ -68,7 +70,6 @@ struct BatchArgs {
   BatchHdr* hdr;
   const BatchClass* classes;
   const BatchQuery* bq;
-  unsigned* blockdone;
   int* qb;
   unsigned long long* cnu_qt;
   unsigned long long* cnu_q;
@@ -87,19 +88,12 @@ struct BatchArgs {
   int D, L, M, Jp, tb, shift;
 };
 
-constexpr unsigned kSpinLimit = 1u << 24;  // ~ seconds of s_sleep before a timeout
-
-// Expands the request list into per-query descriptors; clears the probe
-// block counters.
+// Expands the request list into per-query descriptors.
 __global__ __launch_bounds__(256) void k_batch_expand(const BatchHdr* __restrict__ hdr,
                                                       const BatchReq* __restrict__ reqs,
-                                                      const BatchClass* __restrict__ classes,
-                                                      BatchQuery* __restrict__ bq,
-                                                      unsigned* __restrict__ blockdone,
-                                                      int nblk_cap) {
+                                                      BatchQuery* __restrict__ bq) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  const int Qtot = hdr->Qtot, nreq = hdr->nreq, nclass = hdr->nclass, qn = hdr->qn;
-  if (i < nblk_cap) blockdone[i] = 0u;
+  const int Qtot = hdr->Qtot, nreq = hdr->nreq;
   if (i >= Qtot) return;
   int lo = 0, hi = nreq - 1;  // last request with b0 <= i
   while (lo < hi) {
@@ -110,8 +104,6 @@ __global__ __launch_bounds__(256) void k_batch_expand(const BatchHdr* __restrict
       hi = mid - 1;
   }
   const BatchReq r = reqs[lo];
-  int c = 0;
-  while (c + 1 < nclass && classes[c + 1].b0 <= i) ++c;
   const int j = i - r.b0;
   BatchQuery d;
   d.qrow = r.qrow0 + j;
@@ -119,45 +111,47 @@ __global__ __launch_bounds__(256) void k_batch_expand(const BatchHdr* __restrict
   d.poff = r.p0 + (long long)j * r.T;
   d.K = r.K;
   d.T = r.T;
-  d.blk = classes[c].blk0 + (i - classes[c].b0) / qn;
-  d.pad = 0;
   bq[i] = d;
 }
 
-// One probe item: queries [i0, i0 + nq) of one class (same T), tables
-// [t_begin, t_end), qn x tn pairs per pass.
+// Probe stage of queries [i0, i0 + nq) (same T), all tables, qn x tn pairs
+// per pass.  Query vectors are staged in LDS when they fit beside the octet
+// scratch, else read from global memory.
 template <int J, int MI>
-__device__ inline void batch_probe_item(const BatchArgs& a, const BatchHdr& h, int i0, int nq,
-                                        int t_begin, int t_end, int T, unsigned char* pool,
-                                        int pool_bytes) {
+__device__ __forceinline__ void batch_probe(const BatchArgs& a, const BatchHdr& h, int i0, int nq,
+                                            int T, unsigned char* pool, int pool_bytes) {
   __shared__ long long s_qrow[32];
   const int tid = threadIdx.x, oct = tid >> 3;
-  const int D = a.D, M = a.M, qn = h.qn, tn = h.tn;
-  if (tid < 32) s_qrow[tid] = tid < nq ? a.bq[i0 + tid].qrow : 0;
-  __syncthreads();
+  const int D = a.D, M = a.M, L = a.L, qn = h.qn, tn = h.tn;
   const int ys_bytes = (qn * D * 4 + 15) & ~15;
   const bool stage = ys_bytes + 32 * probe8_octet_bytes(M) <= pool_bytes;
   float* Ys = reinterpret_cast<float*>(pool);
-  if (stage) {
-    for (int e = tid; e < nq * D; e += 256) {
-      const int r = e / D;
-      Ys[e] = h.Qm[s_qrow[r] * D + (e - r * D)];
-    }
-  }
-  __syncthreads();
   unsigned char* orb = pool + (stage ? ys_bytes : 0) + oct * probe8_octet_bytes(M);
   const int qi = oct % qn, ti = oct / qn;
-  const int qa = qi < nq ? qi : 0;
-  const int i = i0 + qa;
-  const float* y = stage ? Ys + qa * D : h.Qm + s_qrow[qa] * D;
-  const long long poff = a.bq[i].poff;
-  for (int tt = t_begin; tt < t_end; tt += tn) {
-    const int t = tt + ti;
-    const bool active = qi < nq && t < t_end;
-    const int te = active ? t : t_begin;
-    probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                       a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * h.Ptot + poff,
-                       a.cnu_qt + (long long)i * a.L + te);
+  for (int s0 = 0; s0 < nq; s0 += qn) {
+    const int ns = min(qn, nq - s0);
+    __syncthreads();  // the previous sub-block's readers of s_qrow / Ys are done
+    if (tid < qn) s_qrow[tid] = tid < ns ? a.bq[i0 + s0 + tid].qrow : 0;
+    __syncthreads();
+    if (stage) {
+      for (int e = tid; e < ns * D; e += 256) {
+        const int r = e / D;
+        Ys[e] = h.Qm[s_qrow[r] * D + (e - r * D)];
+      }
+      __syncthreads();
+    }
+    const int qa = qi < ns ? qi : 0;
+    const int i = i0 + s0 + qa;
+    const float* y = stage ? Ys + qa * D : h.Qm + s_qrow[qa] * D;
+    const long long poff = a.bq[i].poff;
+    for (int tt = 0; tt < L; tt += tn) {
+      const int t = tt + ti;
+      const bool active = qi < ns && t < L;
+      const int te = active ? t : 0;
+      probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+                         a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * h.Ptot + poff,
+                         a.cnu_qt + (long long)i * L + te);
+    }
   }
 }
 
@@ -169,12 +163,10 @@ template <int MI, int R, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
   __shared__ unsigned s_item;
-  __shared__ int s_go;
   static_assert(kUcap + 3 * kScap + kYcap <= kHcap, "post-union LDS layout");
   const int tid = threadIdx.x;
   BatchHdr* hp = a.hdr;
   const BatchHdr h = *hp;
-  const unsigned nitems = (unsigned)h.nprobe + (unsigned)h.Qtot;
   CandBufs B;
   B.hash = s_pool;
   B.hbits = 13;
@@ -191,50 +183,26 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
     if (tid == 0) s_item = atomicAdd(&hp->next_item, 1u);
     __syncthreads();
     const unsigned it = s_item;
-    if (it >= nitems) break;
-    if (it < (unsigned)h.nprobe) {
-      int c = 0;
-      while (c + 1 < h.nclass && a.classes[c + 1].item0 <= (int)it) ++c;
-      const BatchClass C = a.classes[c];
-      const int local = (int)it - C.item0;
-      const int bl = local / h.ntg, g = local - bl * h.ntg;
-      const int i0 = C.b0 + bl * h.qn;
-      const int nq = min(h.qn, C.b0 + C.n - i0);
-      const int t_begin = g * h.tgroup, t_end = min(a.L, t_begin + h.tgroup);
-      unsigned char* pool = reinterpret_cast<unsigned char*>(s_pool);
-      const int pb = (int)sizeof(s_pool);
-      if (C.T <= 8)
-        batch_probe_item<1, MI>(a, h, i0, nq, t_begin, t_end, C.T, pool, pb);
-      else if (C.T <= 16)
-        batch_probe_item<2, MI>(a, h, i0, nq, t_begin, t_end, C.T, pool, pb);
-      else if (C.T <= 32)
-        batch_probe_item<4, MI>(a, h, i0, nq, t_begin, t_end, C.T, pool, pb);
-      else
-        batch_probe_item<8, MI>(a, h, i0, nq, t_begin, t_end, C.T, pool, pb);
-      // Publish: every thread's qb / C_nu stores are visible device-wide
-      // before the block counter moves.
-      __threadfence();
-      __syncthreads();
-      if (tid == 0) atomicAdd(&a.blockdone[C.blk0 + bl], 1u);
-    } else {
-      const int i = (int)(it - (unsigned)h.nprobe);
+    if (it >= (unsigned)h.nitems) break;
+    int c = 0;
+    while (c + 1 < h.nclass && a.classes[c + 1].item0 <= (int)it) ++c;
+    const BatchClass C = a.classes[c];
+    const int i0 = C.b0 + ((int)it - C.item0) * h.bq;
+    const int nq = min(h.bq, C.b0 + C.n - i0);
+    unsigned char* pool = reinterpret_cast<unsigned char*>(s_pool);
+    const int pb = (int)sizeof(s_pool);
+    if (C.T <= 8)
+      batch_probe<1, MI>(a, h, i0, nq, C.T, pool, pb);
+    else if (C.T <= 16)
+      batch_probe<2, MI>(a, h, i0, nq, C.T, pool, pb);
+    else if (C.T <= 32)
+      batch_probe<4, MI>(a, h, i0, nq, C.T, pool, pb);
+    else
+      batch_probe<8, MI>(a, h, i0, nq, C.T, pool, pb);
+    // The block's probe slots and C_nu rows were written by this workgroup;
+    // query_candidates starts with a barrier.
+    for (int i = i0; i < i0 + nq; ++i) {
       const BatchQuery d = a.bq[i];
-      if (tid == 0) {
-        int go = 1;
-        unsigned spins = 0;
-        while (atomicAdd(&a.blockdone[d.blk], 0u) < (unsigned)h.ntg) {
-          __builtin_amdgcn_s_sleep(8);
-          if (++spins == kSpinLimit) {
-            atomicOr(&hp->err, 2);
-            go = 0;
-            break;
-          }
-        }
-        s_go = go;
-      }
-      __syncthreads();
-      __threadfence();  // acquire: the block's probe results
-      if (!s_go) continue;
       QueryIO io;
       io.y = h.Qm + d.qrow * a.D;
       io.qbq = a.qb + d.poff;
@@ -248,6 +216,116 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
                                &hp->n_recheck, B)) {
         if (tid == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
       }
+    }
+  }
+}
+
+// Split schedule, probe stage: probe items (class, block of qn queries, tn
+// tables) -- 32 (query, table) pairs, one pass -- taken grid-stride, items
+// of heavier classes first.  Needs batch_probe_lds(qn, D, M) bytes of
+// dynamic LDS.
+__host__ __device__ inline bool batch_probe_stage(int qn, int D, int M) {
+  return ((qn * D * 4 + 15) & ~15) + 32 * probe8_octet_bytes(M) <= 64 * 1024;
+}
+__host__ __device__ inline int batch_probe_lds(int qn, int D, int M) {
+  return (batch_probe_stage(qn, D, M) ? ((qn * D * 4 + 15) & ~15) : 0) +
+         32 * probe8_octet_bytes(M);
+}
+
+template <int MI>
+__global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ long long s_qrow[32];
+  const int tid = threadIdx.x, oct = tid >> 3;
+  const BatchHdr h = *a.hdr;
+  const int D = a.D, M = a.M, L = a.L, qn = h.qn, tn = h.tn;
+  const int ntg = (L + tn - 1) / tn;
+  const bool stage = batch_probe_stage(qn, D, M);
+  const int ys_bytes = stage ? ((qn * D * 4 + 15) & ~15) : 0;
+  float* Ys = reinterpret_cast<float*>(smem);
+  unsigned char* orb = smem + ys_bytes + oct * probe8_octet_bytes(M);
+  const int qi = oct % qn, ti = oct / qn;
+  for (int it = blockIdx.x; it < h.nprobe; it += gridDim.x) {
+    int c = 0;
+    while (c + 1 < h.nclass && a.classes[c + 1].pitem0 <= it) ++c;
+    const BatchClass C = a.classes[c];
+    const int local = it - C.pitem0;
+    const int bl = local / ntg, g = local - bl * ntg;
+    const int i0 = C.b0 + bl * qn;
+    const int ns = min(qn, C.b0 + C.n - i0);
+    __syncthreads();  // the previous item's readers of s_qrow / Ys are done
+    if (tid < qn) s_qrow[tid] = tid < ns ? a.bq[i0 + tid].qrow : 0;
+    __syncthreads();
+    if (stage) {
+      for (int e = tid; e < ns * D; e += 256) {
+        const int r = e / D;
+        Ys[e] = h.Qm[s_qrow[r] * D + (e - r * D)];
+      }
+      __syncthreads();
+    }
+    const int qa = qi < ns ? qi : 0;
+    const int i = i0 + qa;
+    const float* y = stage ? Ys + qa * D : h.Qm + s_qrow[qa] * D;
+    const long long poff = a.bq[i].poff;
+    const int t = g * tn + ti;
+    const bool active = qi < ns && t < L;
+    const int te = active ? t : 0;
+    int* qbp = a.qb + (long long)te * h.Ptot + poff;
+    unsigned long long* cnu = a.cnu_qt + (long long)i * L + te;
+    const int T = C.T;
+    if (T <= 8)
+      probe8_pair<1, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
+    else if (T <= 16)
+      probe8_pair<2, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
+    else if (T <= 32)
+      probe8_pair<4, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
+    else
+      probe8_pair<8, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
+  }
+}
+
+// Split schedule, candidate stage: one query per work item, claimed from the
+// ticket counter in batch order (heaviest T and K first).
+template <int R>
+__global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
+  __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
+  __shared__ unsigned s_item;
+  const int tid = threadIdx.x;
+  BatchHdr* hp = a.hdr;
+  const BatchHdr h = *hp;
+  CandBufs B;
+  B.hash = s_pool;
+  B.hbits = 13;
+  B.ulist = s_pool + kHcap;
+  B.ucap = kUcap;
+  B.d32 = reinterpret_cast<float*>(s_pool);
+  B.sd = reinterpret_cast<double*>(s_pool + kUcap);
+  B.sid = s_pool + kUcap + 2 * kScap;
+  B.scap = kScap;
+  B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_item = atomicAdd(&hp->next_item, 1u);
+    __syncthreads();
+    const int i = (int)s_item;
+    if (i >= h.Qtot) break;
+    const BatchQuery d = a.bq[i];
+    QueryIO io;
+    io.y = h.Qm + d.qrow * a.D;
+    io.qbq = a.qb + d.poff;
+    io.tstride = h.Ptot;
+    io.T = d.T;
+    io.K = d.K;
+    io.oid = h.out_ids + d.out;
+    io.od = h.out_d + d.out;
+    io.slot = i;
+    if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+                             &hp->n_recheck, B)) {
+      if (tid == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
     }
   }
 }

@@ -78,7 +78,7 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
 // for p < T and *cnu (summed sizes of the matched buckets) when active.
 // Every lane of the octet calls it; T is uniform over the wave.
 template <int J, int MI>
-__device__ inline void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
+__device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
                                    const float* __restrict__ Ad, int Jp,
                                    const double* __restrict__ b64, double W,
                                    const uint64_t* __restrict__ seeds, int tb, int shift,

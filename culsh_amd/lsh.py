@@ -287,9 +287,10 @@ class Batch(_Base):
             self._L.culsh_bt_destroy(self._h)
             self._h = ctypes.c_void_p()
 
-    def set_schedule(self, probe_queries=32, probe_tables=1, tables_per_item=1,
-                     rows_in_flight=4, wgs_per_cu=2):
-        s = Schedule(probe_queries, probe_tables, tables_per_item, rows_in_flight, wgs_per_cu)
+    def set_schedule(self, probe_queries=32, probe_tables=1, queries_per_item=32,
+                     rows_in_flight=4, wgs_per_cu=3, fused=0):
+        s = Schedule(probe_queries, probe_tables, queries_per_item, rows_in_flight, wgs_per_cu,
+                     fused)
         return self._ok(self._L.culsh_bt_set_schedule(self._h, ctypes.byref(s)))
 
     def get_schedule(self):

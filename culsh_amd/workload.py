@@ -113,7 +113,40 @@ CONFIGS = {
                  K=10, T=32, data="gamma_ints", query="same", seed_base=2000),
     "c3": Config("c3-mixture-4M-precision", N=4_000_000, D=96, L=32, M=20, W=2.0, Q=50_000,
                  K=100, T=64, data="mixture", n_centers=1000, sigma=0.05, seed_base=3000),
+    # Serving indexes: (a) 2M x 128 (W chosen for recall@10 >= 0.9), (b) 1M x 960.
+    "c5a": Config("c5a-mixture-2M-serve", N=2_000_000, D=128, L=24, M=20, W=2.5, Q=0, K=10,
+                  T=16, data="mixture", n_centers=1000, sigma=0.05, seed_base=5000),
+    "c5b": Config("c5b-mixture-1M-d960-serve", N=1_000_000, D=960, L=16, M=28, W=3.0, Q=0,
+                  K=10, T=16, data="mixture", n_centers=500, sigma=0.05, seed_base=5100),
 }
+
+
+def log_uniform_q(rng, n, qmax):
+    """Q = floor(exp(U ln(qmax + 1))), U ~ U[0, 1): log-uniform on [1, qmax]."""
+    return np.floor(np.exp(rng.uniform(0.0, np.log(qmax + 1.0), n))).astype(np.int64).clip(1, qmax)
+
+
+def batch_requests(n=256, qmax=4096, Ks=(1, 10, 100), Ts=(1, 8, 64), seed=4004):
+    """The batching workload's requests against the c3 index: (Q, K, T) with Q
+    log-uniform in [1, qmax], K and T uniform over their sets."""
+    rng = np.random.default_rng(seed)
+    Q = log_uniform_q(rng, n, qmax)
+    K = rng.choice(np.asarray(Ks), n)
+    T = rng.choice(np.asarray(Ts), n)
+    return [(int(q), int(k), int(t)) for q, k, t in zip(Q, K, T)]
+
+
+def serve_stream(rate, n, qmax=512, p_a=0.7, Ks=(10, 100), Ts=(16, 64), seed=5004):
+    """The serving workload's request stream: Poisson arrivals at `rate`
+    requests/s; per request (arrival s, index 0=a / 1=b, Q, K, T) with Q
+    log-uniform in [1, qmax]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    idx = (rng.uniform(size=n) >= p_a).astype(np.int64)
+    Q = log_uniform_q(rng, n, qmax)
+    K = rng.choice(np.asarray(Ks), n)
+    T = rng.choice(np.asarray(Ts), n)
+    return [(float(a), int(i), int(q), int(k), int(tt)) for a, i, q, k, tt in zip(t, idx, Q, K, T)]
 
 
 def make_data(cfg, N=None, Q=None):

@@ -145,8 +145,10 @@ int culsh_st_kernel_ms(culsh_st* st, float* probe_ms, float* candidates_ms);
  * Q, K and T against one index; each request's results are exactly those of
  * culsh_st_reset(ht, K, T) + culsh_st_search_device on its queries alone.
  * The whole batch runs as one hipGraph launch: a descriptor copy, one
- * expansion kernel, one persistent fused probe + candidate kernel and one
- * kernel for queries whose candidates overflow on-chip memory. */
+ * expansion kernel, the probe and candidate stages (one persistent fused
+ * kernel, or a probe kernel and a persistent candidate kernel; see
+ * culsh_schedule.fused) and one kernel for queries whose candidates overflow
+ * on-chip memory. */
 typedef struct culsh_bt culsh_bt;
 
 typedef struct {
@@ -160,9 +162,14 @@ typedef struct {
 typedef struct {
   int probe_queries;   /* projection tile: queries per pass (32, 16 or 8) */
   int probe_tables;    /* projection tile: tables per pass (32 / probe_queries) */
-  int tables_per_item; /* probe-group width: tables one probe item covers */
+  int queries_per_item;/* probe-group width: queries per work item (a multiple
+                          of probe_queries, <= 256) */
   int rows_in_flight;  /* candidate chunk: rows per 16-lane group (2, 4, 8) */
   int wgs_per_cu;      /* persistent workgroups per compute unit (1..3) */
+  int fused;           /* 1: one persistent kernel, a work item = a block of
+                          queries (probes over all tables, then candidates);
+                          0: a probe kernel over every request, then a
+                          persistent candidate kernel */
 } culsh_schedule;
 
 int culsh_bt_create(culsh_ht* ht, culsh_bt** out);
