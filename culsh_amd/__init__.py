@@ -5,7 +5,7 @@ kernels in ``lib/libculsh.so`` behind the C ABI of ``include/culsh_c.h``,
 with this package as the Python mirror of the reference classes.
 """
 from .lsh import (FORM_PLAIN, FORM_PRECISION, Batch, HashTables, SearchTables,  # noqa: F401
-                  layout_requests)
+                  Server, layout_requests)
 
-__all__ = ["HashTables", "SearchTables", "Batch", "layout_requests", "FORM_PLAIN",
+__all__ = ["HashTables", "SearchTables", "Batch", "Server", "layout_requests", "FORM_PLAIN",
            "FORM_PRECISION"]

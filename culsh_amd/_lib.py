@@ -30,6 +30,24 @@ class Schedule(ctypes.Structure):
                 ("rows_in_flight", c_int), ("wgs_per_cu", c_int), ("fused", c_int)]
 
 
+class SrvRequest(ctypes.Structure):
+    _fields_ = [("arrival_s", c_double), ("index", c_int), ("Q", c_int), ("K", c_int),
+                ("T", c_int), ("q_row", ctypes.c_longlong), ("out_offset", ctypes.c_longlong)]
+
+
+class SrvConfig(ctypes.Structure):
+    _fields_ = [("slots_per_index", c_int), ("batching", c_int), ("hbm_fraction", c_double),
+                ("max_batch_queries", ctypes.c_longlong)]
+
+
+class SrvStats(ctypes.Structure):
+    _fields_ = [("p50_ms", c_double), ("p99_ms", c_double), ("mean_ms", c_double),
+                ("max_ms", c_double), ("queries_per_s", c_double), ("requests_per_s", c_double),
+                ("batches", ctypes.c_longlong), ("queries", ctypes.c_longlong),
+                ("requests", ctypes.c_longlong), ("mean_batch_requests", c_double),
+                ("slot_bytes", ctypes.c_longlong), ("deferred", ctypes.c_longlong)]
+
+
 # name -> (restype, argtypes); every symbol include/culsh_c.h declares.
 SIGNATURES = {
     "culsh_last_error": (c_char_p, []),
@@ -86,6 +104,10 @@ SIGNATURES = {
     "culsh_bt_rechecked": (c_ull, [c_vp]),
     "culsh_bt_autotune": (c_int, [c_vp, P(Request), c_int, c_vp, c_vp, c_vp, c_int, P(c_float),
                                   P(c_float)]),
+    "culsh_srv_create": (c_int, [P(c_vp), c_int, P(SrvConfig), P(c_vp)]),
+    "culsh_srv_destroy": (None, [c_vp]),
+    "culsh_srv_run": (c_int, [c_vp, P(SrvRequest), c_int, P(c_vp), P(c_vp), P(c_vp), P(c_double),
+                              P(SrvStats)]),
 }
 
 FORM_PLAIN = 0

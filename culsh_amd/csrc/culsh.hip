@@ -1274,3 +1274,6 @@ unsigned long long culsh_st_rechecked(culsh_st* st) { return st ? st->rechecked 
 // Batching form (culsh_bt_*): shares this translation unit's helpers and
 // index state.
 #include "batch_host.hpp"
+
+// Serving form (culsh_srv_*): drives the batching form's plans.
+#include "serve_host.hpp"

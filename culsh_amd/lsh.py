@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FORM_PLAIN, FORM_PRECISION, Request, Schedule, Times  # noqa: F401
+from ._lib import (FORM_PLAIN, FORM_PRECISION, Request, Schedule, SrvConfig,  # noqa: F401
+                   SrvRequest, SrvStats, Times)
 
 
 def _f32(a):
@@ -328,3 +329,48 @@ class Batch(_Base):
                                                   ctypes.byref(a), ctypes.byref(b))):
             return None
         return a.value, b.value
+
+
+class Server(_Base):
+    """Serving form: a real-time request loop over device-resident indexes.
+    batching=True: continuous batching (slots of the batching form, admitted
+    by each slot's share of free HBM); batching=False: one request at a time
+    through the search path (the plain baseline)."""
+
+    def __init__(self, indexes, batching=True, slots_per_index=2, hbm_fraction=0.5,
+                 max_batch_queries=0):
+        self._L = _lib.lib()
+        self._hts = list(indexes)
+        arr = (ctypes.c_void_p * len(self._hts))(*[h.handle.value for h in self._hts])
+        cfg = SrvConfig(int(slots_per_index), int(bool(batching)), float(hbm_fraction),
+                        int(max_batch_queries))
+        h = ctypes.c_void_p()
+        if self._L.culsh_srv_create(arr, len(self._hts), ctypes.byref(cfg), ctypes.byref(h)) != 0:
+            raise RuntimeError(_lib.last_error())
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.culsh_srv_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def run(self, reqs, queries, ids, dists):
+        """reqs: [(arrival_s, index, Q, K, T, q_row, out_offset)] sorted by arrival;
+        queries / ids / dists: per index, device tensors.  Returns
+        (latencies_ms numpy array, stats dict) or None on failure."""
+        n = len(reqs)
+        arr = (SrvRequest * n)()
+        for i, (a, ix, Q, K, T, row, off) in enumerate(reqs):
+            arr[i] = SrvRequest(float(a), int(ix), int(Q), int(K), int(T), int(row), int(off))
+        k = len(self._hts)
+        qp = (ctypes.c_void_p * k)(*[t.data_ptr() for t in queries])
+        ip = (ctypes.c_void_p * k)(*[t.data_ptr() for t in ids])
+        dp = (ctypes.c_void_p * k)(*[t.data_ptr() for t in dists])
+        lat = np.zeros(n, np.float64)
+        st = SrvStats()
+        ok = self._ok(self._L.culsh_srv_run(self._h, arr, n, qp, ip, dp,
+                                            lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            ctypes.byref(st)))
+        if not ok:
+            return None
+        return lat, {f: getattr(st, f) for f, _ in SrvStats._fields_}

@@ -194,6 +194,54 @@ unsigned long long culsh_bt_rechecked(culsh_bt* bt);
 int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
                       int* d_ids, float* d_dists, int reps, float* default_ms, float* tuned_ms);
 
+/* ---- Serving (continuous batching over device-resident indexes) ---------
+ * No counterpart in the reference.  A request loop on the calling thread:
+ * requests arrive at their arrival times, wait in a per-index queue, and are
+ * admitted into the next batch of that index while the batch fits one slot's
+ * share of HBM (the slots' buffers are sized once, from the free HBM when
+ * the server is created).  Every slot holds one culsh_bt; batches of
+ * different slots run concurrently on their own streams.  A request's
+ * results equal a SearchTables search of it alone. */
+typedef struct culsh_srv culsh_srv;
+
+typedef struct {
+  double arrival_s;      /* arrival time, seconds after the run starts */
+  int index;             /* which of the server's indexes */
+  int Q, K, T;           /* T <= 64 */
+  long long q_row;       /* first query row in that index's query array */
+  long long out_offset;  /* first element of its [K x Q] results */
+} culsh_srv_request;
+
+typedef struct {
+  int slots_per_index;    /* concurrent batches per index (>= 1) */
+  int batching;           /* 1: continuous batching; 0: one request at a time
+                             through the search path (the plain baseline) */
+  double hbm_fraction;    /* share of free HBM the slots may use (0 < f <= 1) */
+  long long max_batch_queries; /* per batch (0 = 65536); HBM may lower it */
+} culsh_srv_config;
+
+typedef struct {
+  double p50_ms, p99_ms, mean_ms, max_ms; /* request latency: completion on the
+                                             GPU timeline minus arrival */
+  double queries_per_s;   /* queries / (last completion - first arrival) */
+  double requests_per_s;
+  long long batches, queries, requests;
+  double mean_batch_requests;
+  long long slot_bytes;   /* HBM reserved per slot */
+  long long deferred;     /* admissions postponed for lack of slot memory */
+} culsh_srv_stats;
+
+int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_config* cfg,
+                     culsh_srv** out);
+void culsh_srv_destroy(culsh_srv* srv);
+/* Serves the requests (sorted by arrival) in real time; d_queries[i],
+ * d_ids[i], d_dists[i] are index i's device arrays.  latency_ms (n entries,
+ * may be null) receives each request's latency.  Returns after the last
+ * completion. */
+int culsh_srv_run(culsh_srv* srv, const culsh_srv_request* reqs, int n,
+                  const float* const* d_queries, int* const* d_ids, float* const* d_dists,
+                  double* latency_ms, culsh_srv_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
