@@ -114,7 +114,7 @@ CONFIGS = {
     "c3": Config("c3-mixture-4M-precision", N=4_000_000, D=96, L=32, M=20, W=2.0, Q=50_000,
                  K=100, T=64, data="mixture", n_centers=1000, sigma=0.05, seed_base=3000),
     # Serving indexes: (a) 2M x 128 (W chosen for recall@10 >= 0.9), (b) 1M x 960.
-    "c5a": Config("c5a-mixture-2M-serve", N=2_000_000, D=128, L=24, M=20, W=2.5, Q=0, K=10,
+    "c5a": Config("c5a-mixture-2M-serve", N=2_000_000, D=128, L=24, M=20, W=3.0, Q=0, K=10,
                   T=16, data="mixture", n_centers=1000, sigma=0.05, seed_base=5000),
     "c5b": Config("c5b-mixture-1M-d960-serve", N=1_000_000, D=960, L=16, M=28, W=3.0, Q=0,
                   K=10, T=16, data="mixture", n_centers=500, sigma=0.05, seed_base=5100),
