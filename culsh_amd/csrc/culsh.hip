@@ -363,13 +363,16 @@ int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
   DevBuf<float> nx;
   TRY(nx.ensure(N));
   hipLaunchKernelGGL(k_row_norms, dim3(grid_for(N, 16)), dim3(256), 0, s, ht->dX, N, D, nx.p);
-  const unsigned long long cap = std::max<unsigned long long>(1ull << 20, (unsigned long long)(N * J / 64));
+  // Recheck list: kRcShards shards of `seg` entries, one counter each.
+  const unsigned long long seg =
+      std::max<unsigned long long>(1ull << 20, (unsigned long long)(N * J / 64)) / kRcShards;
+  const unsigned long long cap = seg * kRcShards;
   DevBuf<unsigned long long> list, cnt;
   DevBuf<int> err;
   TRY(list.ensure(cap));
-  TRY(cnt.ensure(1));
+  TRY(cnt.ensure(kRcShards));
   TRY(err.ensure(1));
-  HIP_TRY(hipMemsetAsync(cnt.p, 0, 8, s));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, 8 * kRcShards, s));
   HIP_TRY(hipMemsetAsync(err.p, 0, 4, s));
   const long long nrt = (N + kPjRows - 1) / kPjRows;
   const long long tiles = nrt * (Jp / kPjCols);
@@ -379,26 +382,37 @@ int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
     hipLaunchKernelGGL(k_project_bf16x3, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
                        (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
                        (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p,
-                       cnt.p, cap, err.p);
+                       cnt.p, seg, err.p);
   else
     hipLaunchKernelGGL(k_project_f32, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
                        (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
                        (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p,
-                       cnt.p, cap, err.p);
-  unsigned long long count = 0;
-  HIP_TRY(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s));
+                       cnt.p, seg, err.p);
+  unsigned long long counts[kRcShards];
+  HIP_TRY(hipMemcpyAsync(counts, cnt.p, sizeof(counts), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipGetLastError());
+  unsigned long long count = 0;
+  bool overflow = false;
+  for (int k = 0; k < kRcShards; ++k) {
+    count += counts[k];
+    overflow |= counts[k] > seg;
+  }
   ht->rechecked = count;
-  if (count > cap) {
+  if (overflow) {
     if (dbg) fprintf(dbg, "[HashTables::index]\t%llu boundary projections; full FP64 pass\n", count);
-    hipLaunchKernelGGL(k_project_f64_all, dim3(grid_for(N * J)), dim3(256), 0, s, ht->dX, N, D,
+    const long long t64 = ((N + kP64Tile - 1) / kP64Tile) * ((J + kP64Tile - 1) / kP64Tile);
+    if (t64 >= (1ll << 31)) return fail(CULSH_E_LIMIT, "FP64 projection grid too large");
+    hipLaunchKernelGGL(k_project_f64_tiled, dim3((unsigned)t64), dim3(256), 0, s, ht->dX, N, D,
                        (const float*)ht->dAorig.p, (const float*)ht->dbj.p, ht->W, M, J, codes.p,
                        err.p);
-  } else if (count > 0) {
-    hipLaunchKernelGGL(k_recheck, dim3(grid_for((long long)count)), dim3(256), 0, s, ht->dX, N,
-                       D, (const float*)ht->dAorig.p, (const float*)ht->dbj.p, ht->W, M,
-                       (const unsigned long long*)list.p, count, codes.p, err.p);
+  } else {
+    for (int k = 0; k < kRcShards; ++k)
+      if (counts[k] > 0)
+        hipLaunchKernelGGL(k_recheck, dim3(grid_for((long long)counts[k])), dim3(256), 0, s,
+                           ht->dX, N, D, (const float*)ht->dAorig.p, (const float*)ht->dbj.p,
+                           ht->W, M, (const unsigned long long*)list.p + k * seg, counts[k],
+                           codes.p, err.p);
   }
   int herr = 0;
   HIP_TRY(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));

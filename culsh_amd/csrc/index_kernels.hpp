@@ -51,23 +51,15 @@ __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X
   }
 }
 
-// Records projection (n, j) for FP64 re-evaluation.
-__device__ inline void push_recheck(long long n, int j, unsigned long long* __restrict__ list,
-                                    unsigned long long* __restrict__ count,
-                                    unsigned long long cap) {
-  unsigned long long i = atomicAdd(count, 1ull);
-  if (i < cap) list[i] = ((unsigned long long)n << 24) | (unsigned long long)j;
-}
-
 // Epilogue of a projection tile element: v = (s - b) / W, code = floor(v),
-// with the FP64 recheck decision.  `err_bound` bounds |s - x.a| (the dot
-// product error of the fast path).  Any rounding of the epilogue itself is
-// added below; the decision is taken in FP64.
-__device__ inline void project_epilogue(float s, long long n, int j, long long N, int M, float bb,
-                                        float W, double err_bound, int16_t* __restrict__ codes,
-                                        unsigned long long* __restrict__ list,
-                                        unsigned long long* __restrict__ count,
-                                        unsigned long long cap, int* __restrict__ err) {
+// and whether the element needs FP64 re-evaluation.  `err_bound` bounds
+// |s - x.a| (the dot product error of the fast path).  Any rounding of the
+// epilogue itself is added below; the decision is taken in FP64.  Returns
+// the recheck decision; invalid elements (outside the tile) return false.
+__device__ inline bool project_epilogue(bool valid, float s, long long n, int j, long long N,
+                                        int M, float bb, float W, double err_bound,
+                                        int16_t* __restrict__ codes, int* __restrict__ err) {
+  if (!valid) return false;
   const float v = (s - bb) / W;
   const double dv = (double)v;
   const double f = floor(dv);
@@ -76,12 +68,75 @@ __device__ inline void project_epilogue(float s, long long n, int j, long long N
                      4.0 * u * fabs(dv) + 1e-300;
   const double fr = dv - f;
   const int t = j / M, m = j - t * M;
-  if (fr <= eps || 1.0 - fr <= eps) push_recheck(n, j, list, count, cap);
   if (f < -(double)(kCodeMax - 1) || f > (double)(kCodeMax - 1)) {
     atomicOr(err, 1);
-    return;
+    return false;
   }
   codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
+  return fr <= eps || 1.0 - fr <= eps;
+}
+
+constexpr int kRcShards = 8;  // recheck list shards (one counter each)
+
+// Epilogue of a 64 x 64 projection tile held as MFMA accumulators (16 rows
+// per wave, lane layout of v_mfma_f32_16x16x*): codes, then the flagged
+// (n, j) appended to recheck-list shard blockIdx % kRcShards with one
+// atomic per workgroup.  Shard k holds entries list[k * seg, k * seg +
+// count[k]); a count above seg means the shard overflowed.  `kerr` is the
+// fast path's dot product error factor (bound = kerr ||x|| ||a_j||).  Every
+// thread of the workgroup must call it.
+__device__ inline void projection_tile_epilogue(
+    const floatx4 (&acc)[4], long long n0, int j0, long long N, int J, int M,
+    const float* __restrict__ bj, const float* __restrict__ naj, const float* __restrict__ nx,
+    float W, double kerr, int16_t* __restrict__ codes, unsigned long long* __restrict__ list,
+    unsigned long long* __restrict__ count, unsigned long long seg, int* __restrict__ err) {
+  __shared__ unsigned s_wcnt[4];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  unsigned bits = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = j0 + c * 16 + (lane & 15);
+    const bool jv = j < J;
+    const float bb = jv ? bj[j] : 0.f;
+    const double na = jv ? (double)naj[j] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
+      const bool valid = jv && n < N;
+      if (project_epilogue(valid, acc[c][i], n, j, N, M, bb, W,
+                           valid ? kerr * (double)nx[n] * na : 0.0, codes, err))
+        bits |= 1u << (c * 4 + i);
+    }
+  }
+  // Offsets: inclusive scan of the per-lane counts within the wave, then
+  // per-wave totals, then one atomic for the workgroup.
+  const unsigned lc = (unsigned)__popc(bits);
+  unsigned incl = lc;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) s_wcnt[w] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+    s_base = tot ? atomicAdd(&count[blockIdx.x % kRcShards], (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long off = s_base + (incl - lc);
+  for (int ww = 0; ww < w; ++ww) off += s_wcnt[ww];
+  unsigned long long* shard = list + (unsigned long long)(blockIdx.x % kRcShards) * seg;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const int c = b >> 2, i = b & 3;
+    const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
+    const int j = j0 + c * 16 + (lane & 15);
+    if (off < seg) shard[off] = ((unsigned long long)n << 24) | (unsigned long long)j;
+    ++off;
+  }
 }
 
 // Maps a linear workgroup id to (row tile, column tile) so that the column
@@ -99,6 +154,35 @@ __device__ inline bool xcd_tile(int nct, long long ntiles, long long& rt, int& c
 
 constexpr int kPjRows = 64, kPjCols = 64, kPjKc = 32;
 
+// Stages the 64 x 32 X tile and the 32 x 64 A tile of one projection step
+// in LDS.  Every load is issued from a clamped, always-valid address before
+// any is consumed (no per-element branches), then out-of-range elements are
+// zeroed.
+__device__ inline void stage_tile(const float* __restrict__ X, long long N, int D,
+                                  const float* __restrict__ Ad, int Jp, long long n0, int j0,
+                                  int d0, float (*Xs)[kPjKc + 1], float (*As)[kPjCols + 1]) {
+  const int tid = threadIdx.x;
+  float xv[8], av[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int e = tid + 256 * k;
+    const int r = e >> 5, c = e & 31;
+    const long long n = min(n0 + r, N - 1);
+    const int d = min(d0 + c, D - 1);
+    xv[k] = X[n * D + d];
+    const int ra = e >> 6, ca = e & 63;
+    av[k] = Ad[(long long)min(d0 + ra, D - 1) * Jp + j0 + ca];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int e = tid + 256 * k;
+    const int r = e >> 5, c = e & 31;
+    Xs[r][c] = (n0 + r < N && d0 + c < D) ? xv[k] : 0.f;
+    const int ra = e >> 6, ca = e & 63;
+    As[ra][ca] = (d0 + ra < D) ? av[k] : 0.f;
+  }
+}
+
 // FP32 projection on MFMA (v_mfma_f32_16x16x4_f32): a 64-point x 64-column
 // tile per workgroup, 16 rows per wave, D staged through LDS in chunks of 32.
 // Ad is [Dp][Jp] (column j = t*M + m of table t), zero padded.
@@ -108,7 +192,7 @@ __global__ __launch_bounds__(256) void k_project_f32(
     int M, const float* __restrict__ bj, const float* __restrict__ naj,
     const float* __restrict__ nx, float W, int16_t* __restrict__ codes,
     unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
-    unsigned long long cap, int* __restrict__ err) {
+    unsigned long long seg, int* __restrict__ err) {
   __shared__ float Xs[kPjRows][kPjKc + 1];
   __shared__ float As[kPjKc][kPjCols + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -123,17 +207,7 @@ __global__ __launch_bounds__(256) void k_project_f32(
 #pragma unroll
   for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int d0 = 0; d0 < D; d0 += kPjKc) {
-    for (int e = tid; e < kPjRows * kPjKc; e += 256) {
-      const int r = e / kPjKc, c = e % kPjKc;
-      const long long n = n0 + r;
-      const int d = d0 + c;
-      Xs[r][c] = (n < N && d < D) ? X[n * D + d] : 0.f;
-    }
-    for (int e = tid; e < kPjKc * kPjCols; e += 256) {
-      const int r = e / kPjCols, c = e % kPjCols;
-      const int d = d0 + r;
-      As[r][c] = (d < D) ? Ad[(long long)d * Jp + j0 + c] : 0.f;
-    }
+    stage_tile(X, N, D, Ad, Jp, n0, j0, d0, Xs, As);
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < kPjKc; kk += 4) {
@@ -147,20 +221,8 @@ __global__ __launch_bounds__(256) void k_project_f32(
     __syncthreads();
   }
   const double ka = 2.0 * (double)(D + 2) * (double)kU32;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = j0 + c * 16 + (lane & 15);
-    if (j >= J) continue;
-    const float bb = bj[j];
-    const double na = (double)naj[j];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
-      if (n >= N) continue;
-      project_epilogue(acc[c][i], n, j, N, M, bb, W, ka * (double)nx[n] * na, codes, list, count,
-                       cap, err);
-    }
-  }
+  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, ka, codes, list, count,
+                           seg, err);
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -190,7 +252,7 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
     int M, const float* __restrict__ bj, const float* __restrict__ naj,
     const float* __restrict__ nx, float W, int16_t* __restrict__ codes,
     unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
-    unsigned long long cap, int* __restrict__ err) {
+    unsigned long long seg, int* __restrict__ err) {
   __shared__ float Xs[kPjRows][kPjKc + 1];
   __shared__ float As[kPjKc][kPjCols + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -206,17 +268,7 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
   for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int kb = (lane >> 4) * 8;  // this lane's 8 consecutive k of the 32
   for (int d0 = 0; d0 < D; d0 += kPjKc) {
-    for (int e = tid; e < kPjRows * kPjKc; e += 256) {
-      const int r = e / kPjKc, c = e % kPjKc;
-      const long long n = n0 + r;
-      const int d = d0 + c;
-      Xs[r][c] = (n < N && d < D) ? X[n * D + d] : 0.f;
-    }
-    for (int e = tid; e < kPjKc * kPjCols; e += 256) {
-      const int r = e / kPjCols, c = e % kPjCols;
-      const int d = d0 + r;
-      As[r][c] = (d < D) ? Ad[(long long)d * Jp + j0 + c] : 0.f;
-    }
+    stage_tile(X, N, D, Ad, Jp, n0, j0, d0, Xs, As);
     __syncthreads();
     bf16x8 xh, xl;
 #pragma unroll
@@ -243,33 +295,18 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
     __syncthreads();
   }
   const double kb3 = 2.0 * (3.01 * 0x1.0p-16 + 3.1 * (double)D * 0x1.0p-24);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = j0 + c * 16 + (lane & 15);
-    if (j >= J) continue;
-    const float bb = bj[j];
-    const double na = (double)naj[j];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
-      if (n >= N) continue;
-      project_epilogue(acc[c][i], n, j, N, M, bb, W, kb3 * (double)nx[n] * na, codes, list, count,
-                       cap, err);
-    }
-  }
+  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, kb3, codes, list, count,
+                           seg, err);
 }
 
-// FP64 projection of one (n, j), the oracle's operation order: sequential
-// FMA over d ascending, then (acc - b) / W (cuLSH_Indexing.cu:43-62 in FP64).
-__device__ inline double project64(const float* __restrict__ x, const float* __restrict__ a,
-                                   int D, float b, float W) {
-  double acc = 0.0;
-  for (int d = 0; d < D; ++d) acc = fma((double)x[d], (double)a[d], acc);
-  return (acc - (double)b) / (double)W;
-}
-
-// Re-evaluates the recorded projections in FP64 and patches their codes.
-// Aorig is [L*M][D]: column j of the projection contiguous over d.
+// Re-evaluates the recorded projections in FP64 in the oracle's operation
+// order -- sequential FMA over d ascending, then (acc - b) / W
+// (cuLSH_Indexing.cu:43-62 in FP64) -- and patches their codes.
+// Aorig is [L*M][D]: column j of the projection contiguous over d.  A
+// workgroup takes 256 recorded (n, j); for each 32-dimension chunk, 32
+// lanes per entry load its x and a segments (128-byte runs) into LDS, then
+// thread i extends entry i's sequential FMA chain (the oracle's order).
+constexpr int kRcDc = 32;
 __global__ __launch_bounds__(256) void k_recheck(const float* __restrict__ X, long long N, int D,
                                                  const float* __restrict__ Aorig,
                                                  const float* __restrict__ bj, float W, int M,
@@ -277,13 +314,45 @@ __global__ __launch_bounds__(256) void k_recheck(const float* __restrict__ X, lo
                                                  unsigned long long count,
                                                  int16_t* __restrict__ codes,
                                                  int* __restrict__ err) {
-  const unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
+  __shared__ float xs[256][kRcDc + 1];
+  __shared__ float as[256][kRcDc + 1];
+  __shared__ unsigned long long s_e[256];
+  const int tid = threadIdx.x;
+  const unsigned long long base = (unsigned long long)blockIdx.x * 256;
+  const unsigned long long i = base + tid;
+  s_e[tid] = i < count ? list[i] : ~0ull;
+  const int lane32 = tid & 31, grp = tid >> 5;  // 8 groups of 32 lanes
+  double acc = 0.0;
+  for (int d0 = 0; d0 < D; d0 += kRcDc) {
+    const int dl = min(kRcDc, D - d0);
+    __syncthreads();
+    // 32 rows per group of 32 lanes; every load in flight before any store.
+    float xv[32], av[32];
+    const int dd = min(d0 + lane32, D - 1);
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const unsigned long long e = s_e[grp + 8 * k];
+      const unsigned long long ee = e == ~0ull ? 0ull : e;
+      const long long n = (long long)(ee >> 24);
+      const int j = (int)(ee & 0xFFFFFFull);
+      xv[k] = X[n * D + dd];
+      av[k] = Aorig[(long long)j * D + dd];
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const int r = grp + 8 * k;
+      const bool ok = s_e[r] != ~0ull && lane32 < dl;
+      xs[r][lane32] = ok ? xv[k] : 0.f;
+      as[r][lane32] = ok ? av[k] : 0.f;
+    }
+    __syncthreads();
+    for (int dd = 0; dd < dl; ++dd) acc = fma((double)xs[tid][dd], (double)as[tid][dd], acc);
+  }
   if (i >= count) return;
-  const unsigned long long e = list[i];
+  const unsigned long long e = s_e[tid];
   const long long n = (long long)(e >> 24);
   const int j = (int)(e & 0xFFFFFFull);
-  const double v = project64(X + n * D, Aorig + (long long)j * D, D, bj[j], W);
-  const double f = floor(v);
+  const double f = floor((acc - (double)bj[j]) / (double)W);
   if (f < -(double)(kCodeMax - 1) || f > (double)(kCodeMax - 1)) {
     atomicOr(err, 1);
     return;
@@ -292,26 +361,79 @@ __global__ __launch_bounds__(256) void k_recheck(const float* __restrict__ X, lo
   codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
 }
 
-// FP64 projection of every (n, j): the fallback when the recheck list
-// overflows its capacity.
-__global__ __launch_bounds__(256) void k_project_f64_all(const float* __restrict__ X, long long N,
-                                                         int D, const float* __restrict__ Aorig,
-                                                         const float* __restrict__ bj, float W,
-                                                         int M, int J,
-                                                         int16_t* __restrict__ codes,
-                                                         int* __restrict__ err) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= N * J) return;
-  const long long n = i / J;
-  const int j = (int)(i - n * J);
-  const double v = project64(X + n * D, Aorig + (long long)j * D, D, bj[j], W);
-  const double f = floor(v);
-  if (f < -(double)(kCodeMax - 1) || f > (double)(kCodeMax - 1)) {
-    atomicOr(err, 1);
-    return;
+// FP64 projection of every (n, j) in the oracle's order, tiled: a workgroup
+// computes 64 points x 64 columns, staging 32-dimension chunks of both as
+// doubles in LDS (coalesced row reads); thread (r, c) keeps 4 x 4 sums, each
+// a sequential FMA over d ascending -- bit-identical to k_recheck's, so no
+// recheck follows.  The path for wide rows (large D), where the FP32 error
+// bound flags too many projections (the recheck list overflows).
+constexpr int kP64Tile = 64, kP64Dc = 32;
+__global__ __launch_bounds__(256) void k_project_f64_tiled(const float* __restrict__ X, long long N,
+                                                           int D, const float* __restrict__ Aorig,
+                                                           const float* __restrict__ bj, float W,
+                                                           int M, int J,
+                                                           int16_t* __restrict__ codes,
+                                                           int* __restrict__ err) {
+  __shared__ double xs[kP64Tile][kP64Dc + 1];
+  __shared__ double as[kP64Tile][kP64Dc + 1];
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const int ctiles = (J + kP64Tile - 1) / kP64Tile;
+  const long long n0 = (long long)(blockIdx.x / ctiles) * kP64Tile;
+  const int j0 = (int)(blockIdx.x % ctiles) * kP64Tile;
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+  for (int d0 = 0; d0 < D; d0 += kP64Dc) {
+    const int dl = min(kP64Dc, D - d0);
+    __syncthreads();
+    float xv[8], av[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = tid + 256 * k;
+      const int row = e >> 5, dd = e & 31;
+      const int d = min(d0 + dd, D - 1);
+      xv[k] = X[min(n0 + row, N - 1) * D + d];
+      av[k] = Aorig[(long long)min(j0 + row, J - 1) * D + d];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = tid + 256 * k;
+      const int row = e >> 5, dd = e & 31;
+      xs[row][dd] = (n0 + row < N && dd < dl) ? (double)xv[k] : 0.0;
+      as[row][dd] = (j0 + row < J && dd < dl) ? (double)av[k] : 0.0;
+    }
+    __syncthreads();
+    for (int dd = 0; dd < dl; ++dd) {
+      double xv[4], av[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xv[r] = xs[tr * 4 + r][dd];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) av[c] = as[tc + 16 * c][dd];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(xv[r], av[c], acc[r][c]);
+    }
   }
-  const int t = j / M, m = j - t * M;
-  codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const long long n = n0 + tr * 4 + r;
+    if (n >= N) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = j0 + tc + 16 * c;
+      if (j >= J) continue;
+      const double f = floor((acc[r][c] - (double)bj[j]) / (double)W);
+      if (f < -(double)(kCodeMax - 1) || f > (double)(kCodeMax - 1)) {
+        atomicOr(err, 1);
+        continue;
+      }
+      const int t = j / M, m = j - t * M;
+      codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
+    }
+  }
 }
 
 // Bucket key of every (table, point): keys[t*N + n], vals = n.

@@ -106,6 +106,23 @@ def test_locality_ordered_large_batch():
     assert_search_equal(g, o)
 
 
+@pytest.mark.parametrize("form", [0, 1])
+def test_wide_rows_take_the_fp64_projection(form):
+    """D = 960 and a small W: more than 2^20 projections fall within the
+    FP32 / BF16x3 error bound of a bucket edge, so the index is projected by
+    the tiled FP64 kernel (the oracle's order, no recheck)."""
+    print("wide rows FP64 projection form", form, flush=True)
+    X = wl.mixture(20000, 960, 50, 0.05, 111)
+    Y = wl.queries_near(X, 60, 0.01, 112)
+    L, M, W = 16, 28, 0.25
+    A, b = wl.projection(L, 960, M, W, 113)
+    g = run_gpu(X, Y, A, b, L, M, W, 10, 16, form=form)
+    assert g["ht"].getRechecked() > (1 << 20)
+    o = run_oracle(X, Y, A, b, L, M, W, 10, 16)
+    assert_buckets_equal(g["ht"], o["index"], L)
+    assert_search_equal(g, o)
+
+
 def test_exact_delta_ties():
     """Duplicated projection columns give exactly tied deltas: the delta order
     then follows the reference heap's mechanics."""
