@@ -33,17 +33,40 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "queries/s at fixed recall@10 (chosen vs plain, same GPU); p50/p99 serve latency"
 
 
-def _roofline(kernels):
+PROFILES = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def _measured_traffic(workload_key, kernel):
+    """HBM bytes per dispatch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    copy), or None."""
+    try:
+        with open(PROFILES) as f:
+            t = json.load(f)
+        return int(t[workload_key][kernel]["hbm_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def _roofline(kernels, workload_key=None):
     """kernels: name -> (ms, algorithmic bytes); the dominant one is reported."""
     dom = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_bytes = kernels[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": dom, "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,
-            "other_kernels": {k: {"ms": round(v[0], 4), "algorithmic_bytes": v[1],
-                                  "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 2)}
-                              for k, v in kernels.items() if k != dom}}
+    traffic = _measured_traffic(workload_key, dom) if workload_key else None
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+           "kernel": dom, "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,
+           "other_kernels": {k: {"ms": round(v[0], 4), "algorithmic_bytes": v[1],
+                                 "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 2),
+                                 "traffic": (_measured_traffic(workload_key, k)
+                                             if workload_key else None)}
+                             for k, v in kernels.items() if k != dom}}
+    if traffic is not None:
+        out["traffic_source"] = ("profiles/traffic.json [%s]: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                 "per dispatch, scaled by a 1 GiB copy" % workload_key)
+        out["traffic_GBps"] = round(traffic / (dom_ms * 1e-3) / 1e9, 2)
+    return out
 
 
 def _search_bytes(D, L, T, K, cnu, cu):
@@ -130,8 +153,9 @@ def bench_search(args):
     out["total_candidates"] = {"nonunique": int(cnu.sum()), "unique": int(cu.sum())}
     out["roofline"] = _roofline({
         "k_query_probe8": (probe_ms / args.steps, probe_bytes),
-        "k_candidates": (cand_ms / args.steps, cand_bytes),
-    })
+        "k_candidates" if form == FORM_PLAIN else "k_group_candidates":
+            (cand_ms / args.steps, cand_bytes),
+    }, f"{args.config}/{args.form}")
 
     if args.cpu_queries > 0 or args.recall_queries > 0:
         import oracle
@@ -357,7 +381,20 @@ def main():
                     help="queries in the oracle's timed CPU sample (0 = skip)")
     ap.add_argument("--recall-queries", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (5 times): the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the LSH path here is single-GPU; see the sharding design).", file=sys.stderr)
