@@ -84,6 +84,7 @@ template <int OCC>
 BatchKernel batch_kernel_o(int M, int R) {
   if (M <= 8) return batch_kernel_r<1, OCC>(R);
   if (M <= 16) return batch_kernel_r<2, OCC>(R);
+  if (M <= 24) return batch_kernel_r<3, OCC>(R);
   if (M <= 32) return batch_kernel_r<4, OCC>(R);
   return batch_kernel_r<8, OCC>(R);
 }
@@ -96,6 +97,7 @@ BatchKernel batch_kernel(int M, int R, int wgs_per_cu) {
 BatchKernel batch_probe_kernel(int M) {
   if (M <= 8) return k_batch_probe<1>;
   if (M <= 16) return k_batch_probe<2>;
+  if (M <= 24) return k_batch_probe<3>;
   if (M <= 32) return k_batch_probe<4>;
   return k_batch_probe<8>;
 }

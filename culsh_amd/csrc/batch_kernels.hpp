@@ -148,7 +148,7 @@ __device__ __forceinline__ void batch_probe(const BatchArgs& a, const BatchHdr& 
       const int t = tt + ti;
       const bool active = qi < ns && t < L;
       const int te = active ? t : 0;
-      probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+      probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
                          a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * h.Ptot + poff,
                          a.cnu_qt + (long long)i * L + te);
     }
@@ -156,7 +156,7 @@ __device__ __forceinline__ void batch_probe(const BatchArgs& a, const BatchHdr& 
 }
 
 // The persistent fused kernel.  MI: dimensions per lane of the probe
-// projection (ceil(M / 8) rounded up to a power of two); R: candidate rows
+// projection (ceil(M / 8), rounded up to 1, 2, 3, 4 or 8); R: candidate rows
 // in flight per 16-lane group; OCC: workgroups per CU the register budget is
 // bounded for (2: 256 registers per lane, 3: 168).  T <= 64.
 template <int MI, int R, int OCC>
@@ -223,9 +223,13 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
 // Split schedule, probe stage: probe items (class, block of qn queries, tn
 // tables) -- 32 (query, table) pairs, one pass -- taken grid-stride, items
 // of heavier classes first.  Needs batch_probe_lds(qn, D, M) bytes of
-// dynamic LDS.
+// dynamic LDS.  Query vectors are read through L1, as in k_query_probe8
+// (occupancy over staging).
 __host__ __device__ inline bool batch_probe_stage(int qn, int D, int M) {
-  return ((qn * D * 4 + 15) & ~15) + 32 * probe8_octet_bytes(M) <= 64 * 1024;
+  (void)qn;
+  (void)D;
+  (void)M;
+  return false;
 }
 __host__ __device__ inline int batch_probe_lds(int qn, int D, int M) {
   return (batch_probe_stage(qn, D, M) ? ((qn * D * 4 + 15) & ~15) : 0) +
@@ -274,16 +278,16 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
     unsigned long long* cnu = a.cnu_qt + (long long)i * L + te;
     const int T = C.T;
     if (T <= 8)
-      probe8_pair<1, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+      probe8_pair<1, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
                          a.dir, a.recs, a.bcodes, orb, qbp, cnu);
     else if (T <= 16)
-      probe8_pair<2, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+      probe8_pair<2, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
                          a.dir, a.recs, a.bcodes, orb, qbp, cnu);
     else if (T <= 32)
-      probe8_pair<4, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+      probe8_pair<4, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
                          a.dir, a.recs, a.bcodes, orb, qbp, cnu);
     else
-      probe8_pair<8, MI>(y, active, te, D, M, T, a.Ad, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
+      probe8_pair<8, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
                          a.dir, a.recs, a.bcodes, orb, qbp, cnu);
   }
 }

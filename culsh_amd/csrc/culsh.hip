@@ -1066,6 +1066,8 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
     CULSH_PROBE8(JJ, 1);     \
   else if (M <= 16)          \
     CULSH_PROBE8(JJ, 2);     \
+  else if (M <= 24)          \
+    CULSH_PROBE8(JJ, 3);     \
   else if (M <= 32)          \
     CULSH_PROBE8(JJ, 4);     \
   else                       \

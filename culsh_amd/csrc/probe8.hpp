@@ -17,9 +17,18 @@ namespace culsh {
 __host__ __device__ inline int probe8_octet_bytes(int M) {
   return (16 * M + 64 + 128 + 4 * M + 15) & ~15;
 }
-// Dynamic LDS of one workgroup: 32 query vectors + 32 octet scratches.
+// Dimensions per lane of the projection: ceil(M / 8) rounded up to one of
+// the instantiated widths 1, 2, 3, 4, 8.
+__host__ __device__ inline int probe8_mi(int M) {
+  const int m = (M + 7) / 8;
+  return m <= 4 ? m : 8;
+}
+// Dynamic LDS of one workgroup: 32 octet scratches.  Query vectors are read
+// through L1 (the 8 lanes of an octet read the same address): staging them
+// in LDS would cost more occupancy than it saves.
 __host__ __device__ inline int probe8_lds_bytes(int D, int M) {
-  return ((32 * D * 4 + 15) & ~15) + 32 * probe8_octet_bytes(M);
+  (void)D;
+  return 32 * probe8_octet_bytes(M);
 }
 
 // Does the probe code of `mask` (base code plus the mask's +-1 moves) equal
@@ -68,18 +77,21 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
 // One (query, table) pair per octet: FP64 projection of y (the oracle's
 // order), deltas, delta order, the multiprobe queue, keys, lookup and verify.
 // J = ceil(T / 8) queue slots per lane (slot s: lane s%8, register s/8);
-// MI >= ceil(M / 8) dimensions per lane (lane l owns l + 8i), a compile-time
-// bound so that the projection loop is branch-free.  Ad is the [D][Jp] f32
-// projection matrix (column t*M + m) with at least 8*MI floats of padding
-// after its last row, read through L1 (lanes past M read padding or the next
-// table's columns and discard them); the FP64 product of f32 operands is
-// exact, so widening on read is the oracle's arithmetic.  orb is the octet's
+// MI >= ceil(M / 8) dimensions per lane (lane l owns l + 8i; instantiated
+// for MI = 1, 2, 3, 4, 8), a compile-time
+// bound so that the projection loop is branch-free.  acol points at the
+// table's column 0 of an f32 projection matrix with row stride astride
+// (column m of row d at acol[d * astride + m]) and at least 8*MI readable
+// columns per row (lanes past M read them and discard them): the [D][Jp]
+// matrix in global memory, or the table's slice staged in LDS.  The FP64
+// product of f32 operands is exact, so widening on read is the oracle's
+// arithmetic.  orb is the octet's
 // probe8_octet_bytes(M) of LDS.  Writes qbp[p] (bucket of probe p, or -1)
 // for p < T and *cnu (summed sizes of the matched buckets) when active.
 // Every lane of the octet calls it; T is uniform over the wave.
 template <int J, int MI>
 __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
-                                   const float* __restrict__ Ad, int Jp,
+                                   const float* __restrict__ acol0, int astride,
                                    const double* __restrict__ b64, double W,
                                    const uint64_t* __restrict__ seeds, int tb, int shift,
                                    const uint64_t* __restrict__ dir,
@@ -102,11 +114,11 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     valid[i] = lane8 + 8 * i < M;
     acc[i] = 0.0;
   }
-  const float* acol = Ad + t * M + lane8;
+  const float* acol = acol0 + lane8;
 #pragma unroll 8
   for (int d = 0; d < D; ++d) {
     const double yd = (double)y[d];
-    const float* arow = acol + (long long)d * Jp;
+    const float* arow = acol + (long long)d * astride;
 #pragma unroll
     for (int i = 0; i < MI; ++i) acc[i] = fma(yd, (double)arow[8 * i], acc[i]);
   }
@@ -290,9 +302,9 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
   if (active && lane8 == 0) *cnu = sizesum;
 }
 
-// The search's octet kernel: 32 queries of one table per workgroup, query
-// vectors staged in LDS.  qb[t][q*T + p], cnu_qt[q*L + t].  Needs
-// probe8_lds_bytes(D, M) bytes of dynamic LDS.
+// The search's octet kernel: 32 queries of one table per workgroup.
+// qb[t][q*T + p], cnu_qt[q*L + t].  Needs probe8_lds_bytes(D, M) bytes of
+// dynamic LDS.
 template <int J, int MI>
 __global__ __launch_bounds__(256) void k_query_probe8(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
@@ -306,16 +318,13 @@ __global__ __launch_bounds__(256) void k_query_probe8(
   const int q0 = (blockIdx.x / L) * 32;
   const int q = q0 + oct;
   const bool active = q < Q;
-  float* Ys = reinterpret_cast<float*>(smem);
-  unsigned char* orb = smem + ((32 * D * 4 + 15) & ~15) + oct * probe8_octet_bytes(M);
-  // Stage the block's query vectors (coalesced).
-  const int nq = min(32, Q - q0);
-  for (int e = tid; e < nq * D; e += 256) Ys[e] = Qm[(long long)q0 * D + e];
-  __syncthreads();
+  unsigned char* orb = smem + oct * probe8_octet_bytes(M);
+  // Query vectors and projection columns are read through L1 (Ad has at
+  // least 64 floats of padding after its last row).
   const int qa = active ? q : q0;
-  probe8_pair<J, MI>(Ys + (active ? oct : 0) * D, active, t, D, M, T, Ad, Jp, b64, W, seeds, tb,
-                     shift, dir, recs, bcodes, orb, qb + (long long)t * Q * T + (long long)qa * T,
-                     cnu_qt + (long long)qa * L + t);
+  probe8_pair<J, MI>(Qm + (long long)qa * D, active, t, D, M, T, Ad + t * M, Jp, b64, W,
+                     seeds, tb, shift, dir, recs, bcodes, orb,
+                     qb + (long long)t * Q * T + (long long)qa * T, cnu_qt + (long long)qa * L + t);
 }
 
 }  // namespace culsh
