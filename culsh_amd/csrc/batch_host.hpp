@@ -94,13 +94,17 @@ BatchKernel batch_kernel(int M, int R, int wgs_per_cu) {
   return wgs_per_cu >= 3 ? batch_kernel_o<3>(M, R) : batch_kernel_o<2>(M, R);
 }
 
-BatchKernel batch_probe_kernel(int M) {
-  if (M <= 8) return k_batch_probe<1>;
-  if (M <= 16) return k_batch_probe<2>;
-  if (M <= 24) return k_batch_probe<3>;
-  if (M <= 32) return k_batch_probe<4>;
-  return k_batch_probe<8>;
+template <int J>
+BatchKernel batch_probe_kernel_j(int M) {
+  if (M <= 8) return k_batch_probe<1, J>;
+  if (M <= 16) return k_batch_probe<2, J>;
+  if (M <= 24) return k_batch_probe<3, J>;
+  if (M <= 32) return k_batch_probe<4, J>;
+  return k_batch_probe<8, J>;
 }
+
+// Queue width index of a probe count: J = 1, 2, 4, 8 slots per lane.
+int batch_ji(int T) { return T <= 8 ? 0 : T <= 16 ? 1 : T <= 32 ? 2 : 3; }
 
 BatchKernel batch_cand_kernel(int R) {
   if (R == 2) return k_batch_cand<2>;
@@ -193,6 +197,12 @@ int bt_capture(culsh_bt* bt) {
   const BatchArgs a = batch_args(bt);
   const int grid = bt->ncu * s.wgs_per_cu;
   const int egrid = (int)grid_for(bt->Qcap);
+  // Split schedule: one workgroup per probe item, as many as a batch within
+  // the plan's capacities can have (every class adds at most one partial
+  // block).
+  const long long pcap = ((long long)(bt->Qcap + s.probe_queries - 1) / s.probe_queries +
+                          bt->Rcap) * ((ht->L + s.probe_tables - 1) / s.probe_tables);
+  if (pcap >= (1ll << 31)) return fail(CULSH_E_LIMIT, "batch probe grid too large");
   bt->drop_graph();
   HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));
   const hipError_t m1 =
@@ -203,8 +213,13 @@ int bt_capture(culsh_bt* bt) {
     hipLaunchKernelGGL(batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu), dim3(grid),
                        dim3(256), 0, bt->cap, a);
   } else {
-    hipLaunchKernelGGL(batch_probe_kernel(ht->M), dim3(bt->ncu * 8), dim3(256),
-                       batch_probe_lds(s.probe_queries, ht->D, ht->M), bt->cap, a);
+    // Widest queues first; a launch whose range is empty exits at once.
+    const int plds = batch_probe_lds(ht->M);
+    const dim3 pg((unsigned)pcap);
+    hipLaunchKernelGGL(batch_probe_kernel_j<8>(ht->M), pg, dim3(256), plds, bt->cap, a);
+    hipLaunchKernelGGL(batch_probe_kernel_j<4>(ht->M), pg, dim3(256), plds, bt->cap, a);
+    hipLaunchKernelGGL(batch_probe_kernel_j<2>(ht->M), pg, dim3(256), plds, bt->cap, a);
+    hipLaunchKernelGGL(batch_probe_kernel_j<1>(ht->M), pg, dim3(256), plds, bt->cap, a);
     hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0, bt->cap,
                        a);
   }
@@ -317,6 +332,19 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
   h.Qtot = b;
   h.nitems = item;
   h.nprobe = pitem;
+  // Classes come in T-descending order, so each queue width's probe items
+  // are one contiguous range.
+  for (int k = 0; k < 4; ++k) {
+    h.pj_lo[k] = 0;
+    h.pj_hi[k] = 0;
+  }
+  for (size_t c = 0; c < vc.size(); ++c) {
+    const int k = batch_ji(vc[c].T);
+    const int lo = vc[c].pitem0;
+    const int hi = c + 1 < vc.size() ? vc[c + 1].pitem0 : pitem;
+    if (h.pj_hi[k] == h.pj_lo[k]) h.pj_lo[k] = lo;
+    h.pj_hi[k] = hi;
+  }
   h.qn = s.probe_queries;
   h.tn = s.probe_tables;
   h.bq = bq;

@@ -50,9 +50,11 @@ struct BatchHdr {
   long long Ptot;   // probe slots per table
   int nreq, nclass, Qtot, nitems;  // nitems: fused work items
   int nprobe;                      // split-schedule probe items
+  int pj_lo[4], pj_hi[4];          // their ranges per queue width J = 1, 2, 4, 8
   int qn, tn, bq;  // schedule: probe tile qn x tn, queries per fused block
   // Device-written counters (zero in the host copy).
-  unsigned next_item;
+  unsigned next_item;      // fused schedule's work queue
+  unsigned next_cand[8];   // split schedule's candidate queue, 8 interleaved shards
   int n_overflow;
   int err;  // bit 0: a query exceeds the overflow path
   int pad1;
@@ -221,86 +223,87 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
 }
 
 // Split schedule, probe stage: probe items (class, block of qn queries, tn
-// tables) -- 32 (query, table) pairs, one pass -- taken grid-stride, items
-// of heavier classes first.  Needs batch_probe_lds(qn, D, M) bytes of
-// dynamic LDS.  Query vectors are read through L1, as in k_query_probe8
-// (occupancy over staging).
-__host__ __device__ inline bool batch_probe_stage(int qn, int D, int M) {
-  (void)qn;
-  (void)D;
-  (void)M;
-  return false;
-}
-__host__ __device__ inline int batch_probe_lds(int qn, int D, int M) {
-  return (batch_probe_stage(qn, D, M) ? ((qn * D * 4 + 15) & ~15) : 0) +
-         32 * probe8_octet_bytes(M);
-}
+// tables) -- 32 (query, table) pairs, one pass -- one per workgroup, items
+// of heavier classes first.  (A grid-stride loop over items keeps the
+// projection's values live across iterations and doubles the registers.)  Needs batch_probe_lds(M) bytes of dynamic LDS.
+// Query vectors are read through L1, as in k_query_probe8.
+__host__ __device__ inline int batch_probe_lds(int M) { return 32 * probe8_octet_bytes(M); }
 
-template <int MI>
+// One launch per queue width J (index ji: J = 1 << ji) over that width's
+// contiguous item range, so each launch has the registers of its own J; the
+// grid covers the plan's largest possible range and the workgroups past
+// this batch's range exit at once.
+template <int MI, int J>
 __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
+  constexpr int ji = J == 1 ? 0 : J == 2 ? 1 : J == 4 ? 2 : 3;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ long long s_qrow[32];
   const int tid = threadIdx.x, oct = tid >> 3;
-  const BatchHdr h = *a.hdr;
-  const int D = a.D, M = a.M, L = a.L, qn = h.qn, tn = h.tn;
+  // Only the header fields this kernel needs (a full copy would crowd the
+  // scalar registers).
+  const BatchHdr* hp = a.hdr;
+  const float* Qm = hp->Qm;
+  const long long Ptot = hp->Ptot;
+  const int nclass = hp->nclass, qn = hp->qn, tn = hp->tn;
+  const int it_lo = hp->pj_lo[ji], it_hi = hp->pj_hi[ji];
+  const int D = a.D, M = a.M, L = a.L;
   const int ntg = (L + tn - 1) / tn;
-  const bool stage = batch_probe_stage(qn, D, M);
-  const int ys_bytes = stage ? ((qn * D * 4 + 15) & ~15) : 0;
-  float* Ys = reinterpret_cast<float*>(smem);
-  unsigned char* orb = smem + ys_bytes + oct * probe8_octet_bytes(M);
+  unsigned char* orb = smem + oct * probe8_octet_bytes(M);
   const int qi = oct % qn, ti = oct / qn;
-  for (int it = blockIdx.x; it < h.nprobe; it += gridDim.x) {
+  {
+    const int it = it_lo + (int)blockIdx.x;
+    if (it >= it_hi) return;
     int c = 0;
-    while (c + 1 < h.nclass && a.classes[c + 1].pitem0 <= it) ++c;
+    while (c + 1 < nclass && a.classes[c + 1].pitem0 <= it) ++c;
     const BatchClass C = a.classes[c];
     const int local = it - C.pitem0;
     const int bl = local / ntg, g = local - bl * ntg;
     const int i0 = C.b0 + bl * qn;
     const int ns = min(qn, C.b0 + C.n - i0);
-    __syncthreads();  // the previous item's readers of s_qrow / Ys are done
     if (tid < qn) s_qrow[tid] = tid < ns ? a.bq[i0 + tid].qrow : 0;
     __syncthreads();
-    if (stage) {
-      for (int e = tid; e < ns * D; e += 256) {
-        const int r = e / D;
-        Ys[e] = h.Qm[s_qrow[r] * D + (e - r * D)];
-      }
-      __syncthreads();
-    }
     const int qa = qi < ns ? qi : 0;
     const int i = i0 + qa;
-    const float* y = stage ? Ys + qa * D : h.Qm + s_qrow[qa] * D;
+    const float* y = Qm + s_qrow[qa] * D;
     const long long poff = a.bq[i].poff;
     const int t = g * tn + ti;
     const bool active = qi < ns && t < L;
     const int te = active ? t : 0;
-    int* qbp = a.qb + (long long)te * h.Ptot + poff;
-    unsigned long long* cnu = a.cnu_qt + (long long)i * L + te;
-    const int T = C.T;
-    if (T <= 8)
-      probe8_pair<1, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
-    else if (T <= 16)
-      probe8_pair<2, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
-    else if (T <= 32)
-      probe8_pair<4, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
-    else
-      probe8_pair<8, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                         a.dir, a.recs, a.bcodes, orb, qbp, cnu);
+    probe8_pair<J, MI>(y, active, te, D, M, C.T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb,
+                       a.shift, a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * Ptot + poff,
+                       a.cnu_qt + (long long)i * L + te);
   }
 }
 
+// Claims the next query of the split schedule's candidate queue: shard x
+// holds batch indices x, x + 8, ... (heaviest first); a workgroup starts on
+// shard blockIdx % 8 and moves on to the next shard when one runs dry, so
+// the queue's atomics spread over 8 counters.  Returns -1 when every shard
+// is exhausted.  Called by thread 0.
+__device__ inline int claim_cand(BatchHdr* hp, int Qtot, int& shard, int& tried) {
+  while (tried < 8) {
+    const unsigned t = atomicAdd(&hp->next_cand[shard], 1u);
+    const long long i = (long long)t * 8 + shard;
+    if (i < Qtot) return (int)i;
+    shard = (shard + 1) & 7;
+    ++tried;
+  }
+  return -1;
+}
+
 // Split schedule, candidate stage: one query per work item, claimed from the
-// ticket counter in batch order (heaviest T and K first).
+// sharded queue in batch order (heaviest T and K first).
 template <int R>
 __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
   __shared__ unsigned s_item;
   const int tid = threadIdx.x;
   BatchHdr* hp = a.hdr;
-  const BatchHdr h = *hp;
+  const float* Qm = hp->Qm;
+  int* out_ids = hp->out_ids;
+  float* out_d = hp->out_d;
+  const long long Ptot = hp->Ptot;
+  const int Qtot = hp->Qtot;
   CandBufs B;
   B.hash = s_pool;
   B.hbits = 13;
@@ -311,21 +314,22 @@ __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
   B.sid = s_pool + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  int shard = (int)(blockIdx.x & 7), tried = 0;
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_item = atomicAdd(&hp->next_item, 1u);
+    if (tid == 0) s_item = (unsigned)claim_cand(hp, Qtot, shard, tried);
     __syncthreads();
     const int i = (int)s_item;
-    if (i >= h.Qtot) break;
+    if (i < 0) break;
     const BatchQuery d = a.bq[i];
     QueryIO io;
-    io.y = h.Qm + d.qrow * a.D;
+    io.y = Qm + d.qrow * a.D;
     io.qbq = a.qb + d.poff;
-    io.tstride = h.Ptot;
+    io.tstride = Ptot;
     io.T = d.T;
     io.K = d.K;
-    io.oid = h.out_ids + d.out;
-    io.od = h.out_d + d.out;
+    io.oid = out_ids + d.out;
+    io.od = out_d + d.out;
     io.slot = i;
     if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
                              &hp->n_recheck, B)) {
