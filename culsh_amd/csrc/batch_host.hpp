@@ -18,7 +18,8 @@ struct culsh_bt {
   unsigned char* hpack = nullptr;  // pinned host image of dpack
   BatchHdr* hres = nullptr;        // pinned: header after the batch
   DevBuf<BatchQuery> dbq;
-  DevBuf<int> dqb, dovl;
+  DevBuf<int> dqb, dovl, dorder;
+  DevBuf<unsigned> dlhist, dlkey;
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q;
   DevBuf<uint32_t> dscr;
   static constexpr int kOvGrid = 16;
@@ -123,6 +124,9 @@ BatchArgs batch_args(culsh_bt* bt) {
   a.cnu_q = bt->dcnu_q.p;
   a.cu_q = bt->dcu_q.p;
   a.ovl = bt->dovl.p;
+  a.lhist = bt->dlhist.p;
+  a.order = bt->dorder.p;
+  a.lkey = bt->dlkey.p;
   a.X = ht->dX;
   a.Ad = ht->dAd.p;
   a.b64 = ht->db64.p;
@@ -168,6 +172,8 @@ int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
     TRY(bt->dcnu_q.exact(Qc));
     TRY(bt->dcu_q.exact(Qc));
     TRY(bt->dovl.exact(Qc));
+    TRY(bt->dorder.exact(Qc));
+    TRY(bt->dlkey.exact(Qc));
     bt->Qcap = Qc;
     moved = true;
   }
@@ -177,6 +183,10 @@ int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
       return fail(CULSH_E_LIMIT, "L * (sum of Q*T over the batch) exceeds 2^31 - 1");
     TRY(bt->dqb.exact((size_t)L * Pc));
     bt->Pcap = Pc;
+    moved = true;
+  }
+  if (!bt->dlhist.p) {
+    TRY(bt->dlhist.exact(kLocBins));
     moved = true;
   }
   if (!bt->dscr.p) {
@@ -220,6 +230,11 @@ int bt_capture(culsh_bt* bt) {
     hipLaunchKernelGGL(batch_probe_kernel_j<4>(ht->M), pg, dim3(256), plds, bt->cap, a);
     hipLaunchKernelGGL(batch_probe_kernel_j<2>(ht->M), pg, dim3(256), plds, bt->cap, a);
     hipLaunchKernelGGL(batch_probe_kernel_j<1>(ht->M), pg, dim3(256), plds, bt->cap, a);
+    // Locality order of the candidate stage (counting sort, fixed grids).
+    hipLaunchKernelGGL(k_batch_lzero, dim3(64), dim3(256), 0, bt->cap, a);
+    hipLaunchKernelGGL(k_batch_lhist, dim3(egrid), dim3(256), 0, bt->cap, a);
+    hipLaunchKernelGGL(k_batch_lscan, dim3(1), dim3(256), 0, bt->cap, a);
+    hipLaunchKernelGGL(k_batch_lscatter, dim3(egrid), dim3(256), 0, bt->cap, a);
     hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0, bt->cap,
                        a);
   }

@@ -22,6 +22,7 @@
 #include "candidates.hpp"
 #include "common.hpp"
 #include "probe8.hpp"
+#include "sort.hpp"
 
 namespace culsh {
 
@@ -77,6 +78,9 @@ struct BatchArgs {
   unsigned long long* cnu_q;
   unsigned long long* cu_q;
   int* ovl;
+  unsigned* lhist;  // [kLocBins] locality histogram, then bin offsets
+  unsigned* lkey;   // [Qtot] locality keys
+  int* order;       // [Qtot] candidate-stage order (locality within class)
   // Index.
   const float* X;
   const float* Ad;
@@ -275,24 +279,95 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
   }
 }
 
-// Claims the next query of the split schedule's candidate queue: shard x
-// holds batch indices x, x + 8, ... (heaviest first); a workgroup starts on
-// shard blockIdx % 8 and moves on to the next shard when one runs dry, so
-// the queue's atomics spread over 8 counters.  Returns -1 when every shard
-// is exhausted.  Called by thread 0.
-__device__ inline int claim_cand(BatchHdr* hp, int Qtot, int& shard, int& tried) {
+// Locality order of the split schedule's candidate stage: a counting sort
+// of the key (class, 12-bit hash of the query's coarse cell: floor((a_m.y -
+// b_m) / 4W) of the first min(6, M) hashes of table 0) -- queries in one
+// coarse cell share most of their candidates, and the class (T) keeps heavy
+// queries first.  Grids are fixed (graph capture); bins beyond the batch stay
+// empty.  The order within a bin is whatever the atomics give: results do
+// not depend on the order.
+constexpr int kLocBins = 1 << 16;
+constexpr int kLocChunk = 16;  // consecutive ordered queries per XCD claim
+
+__device__ inline unsigned batch_loc_key(const BatchArgs& a, const BatchHdr* hp, int i) {
+  const int nclass = hp->nclass;
+  int c = 0;
+  while (c + 1 < nclass && a.classes[c + 1].b0 <= i) ++c;
+  const float* y = hp->Qm + a.bq[i].qrow * a.D;
+  const int mm = a.M < 6 ? a.M : 6;
+  uint64_t cell = 0;
+  for (int m = 0; m < mm; ++m) {
+    float s = 0.f;
+    for (int d = 0; d < a.D; ++d) s = fmaf(y[d], a.Ad[(long long)d * a.Jp + m], s);
+    const int cm = (int)floorf((s - (float)a.b64[m]) / (4.f * (float)a.W));
+    cell = cell * 0x100000001B3ull + (uint64_t)(uint32_t)cm;
+  }
+  const unsigned b = (unsigned)(mix64(cell) >> 52);
+  return ((unsigned)min(c, 15) << 12) | b;
+}
+
+__global__ __launch_bounds__(256) void k_batch_lzero(BatchArgs a) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < kLocBins; i += gridDim.x * 256)
+    a.lhist[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_batch_lhist(BatchArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.hdr->Qtot) return;
+  const unsigned k = batch_loc_key(a, a.hdr, i);
+  a.lkey[i] = k;
+  atomicAdd(&a.lhist[k], 1u);
+}
+
+// Exclusive scan of the kLocBins counts in place (one workgroup, 256
+// consecutive bins per thread).
+__global__ __launch_bounds__(256) void k_batch_lscan(BatchArgs a) {
+  __shared__ unsigned s_misc[4];
+  const int tid = threadIdx.x;
+  constexpr int per = kLocBins / 256;
+  unsigned* h = a.lhist + tid * per;
+  unsigned sum = 0;
+  for (int k = 0; k < per; ++k) sum += h[k];
+  unsigned tot;
+  unsigned run = block_exclusive_scan_256(sum, s_misc, tot);
+  for (int k = 0; k < per; ++k) {
+    const unsigned v = h[k];
+    h[k] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_lscatter(BatchArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.hdr->Qtot) return;
+  const unsigned pos = atomicAdd(&a.lhist[a.lkey[i]], 1u);
+  a.order[pos] = i;
+}
+
+// Claims the next query of the split schedule's candidate queue.  The
+// locality order is cut into chunks of kLocChunk; shard x (XCD x: workgroups
+// x, x + 8, ...) takes chunks x, x + 8, ... in order, so neighbouring
+// queries share an L2, and moves on to the next shard when its own runs dry
+// (8 counters spread the queue's atomics).  Returns -1 when every shard is
+// exhausted.  Called by thread 0.
+__device__ inline int claim_cand(const BatchArgs& a, BatchHdr* hp, int Qtot, int& shard,
+                                 int& tried) {
   while (tried < 8) {
     const unsigned t = atomicAdd(&hp->next_cand[shard], 1u);
-    const long long i = (long long)t * 8 + shard;
-    if (i < Qtot) return (int)i;
-    shard = (shard + 1) & 7;
-    ++tried;
+    const long long p = ((long long)(t / kLocChunk) * 8 + shard) * kLocChunk + t % kLocChunk;
+    if (p < Qtot) return a.order[p];
+    // Chunks of this shard beyond Qtot: it is exhausted once its chunk
+    // start passes the end.
+    if (((long long)(t / kLocChunk) * 8 + shard) * kLocChunk >= Qtot) {
+      shard = (shard + 1) & 7;
+      ++tried;
+    }
   }
   return -1;
 }
 
 // Split schedule, candidate stage: one query per work item, claimed from the
-// sharded queue in batch order (heaviest T and K first).
+// sharded queue in locality order (heaviest classes first).
 template <int R>
 __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
@@ -317,7 +392,7 @@ __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
   int shard = (int)(blockIdx.x & 7), tried = 0;
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_item = (unsigned)claim_cand(hp, Qtot, shard, tried);
+    if (tid == 0) s_item = (unsigned)claim_cand(a, hp, Qtot, shard, tried);
     __syncthreads();
     const int i = (int)s_item;
     if (i < 0) break;

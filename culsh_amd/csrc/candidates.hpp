@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_candidates(
     if (pos >= Q) return;
     q = (int)order[pos];
   }
-  if (!query_candidates(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
+  if (!query_candidates<8>(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
                         cnu_qt, cnu_q, cu_q, n_recheck, B)) {
     if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1)] = q;
   }
