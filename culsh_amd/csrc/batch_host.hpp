@@ -67,7 +67,8 @@ int check_schedule(const culsh_schedule& s) {
                 "queries_per_item must be a multiple of probe_queries, at most 256");
   if (s.rows_in_flight != 2 && s.rows_in_flight != 4 && s.rows_in_flight != 8)
     return fail(CULSH_E_ARG, "rows_in_flight must be 2, 4 or 8");
-  if (s.wgs_per_cu < 1 || s.wgs_per_cu > 3) return fail(CULSH_E_ARG, "wgs_per_cu must be 1..3");
+  if (s.wgs_per_cu < 0 || s.wgs_per_cu > 3 || (s.fused && s.wgs_per_cu == 0))
+    return fail(CULSH_E_ARG, "wgs_per_cu must be 1..3 (0: one workgroup per query, split only)");
   if (s.fused != 0 && s.fused != 1) return fail(CULSH_E_ARG, "fused must be 0 or 1");
   return CULSH_OK;
 }
@@ -111,6 +112,12 @@ BatchKernel batch_cand_kernel(int R) {
   if (R == 2) return k_batch_cand<2>;
   if (R == 8) return k_batch_cand<8>;
   return k_batch_cand<4>;
+}
+
+BatchKernel batch_cand_each_kernel(int R) {
+  if (R == 2) return k_batch_cand_each<2>;
+  if (R == 8) return k_batch_cand_each<8>;
+  return k_batch_cand_each<4>;
 }
 
 BatchArgs batch_args(culsh_bt* bt) {
@@ -235,8 +242,15 @@ int bt_capture(culsh_bt* bt) {
     hipLaunchKernelGGL(k_batch_lhist, dim3(egrid), dim3(256), 0, bt->cap, a);
     hipLaunchKernelGGL(k_batch_lscan, dim3(1), dim3(256), 0, bt->cap, a);
     hipLaunchKernelGGL(k_batch_lscatter, dim3(egrid), dim3(256), 0, bt->cap, a);
-    hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0, bt->cap,
-                       a);
+    if (s.wgs_per_cu == 0) {
+      // One workgroup per query position, chunks interleaved over the XCDs.
+      const long long per = (bt->Qcap + 8ll * kLocChunk - 1) / (8ll * kLocChunk) * kLocChunk;
+      hipLaunchKernelGGL(batch_cand_each_kernel(s.rows_in_flight), dim3((unsigned)(8 * per)),
+                         dim3(256), 0, bt->cap, a);
+    } else {
+      hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0,
+                         bt->cap, a);
+    }
   }
   hipLaunchKernelGGL(k_batch_overflow, dim3(culsh_bt::kOvGrid), dim3(256), 0, bt->cap, a,
                      bt->dscr.p, culsh_bt::kHbitsMax, culsh_bt::kUcapMax);
@@ -456,7 +470,7 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
   const int tiles[3][2] = {{32, 1}, {16, 2}, {8, 4}};
   const int qpis[4] = {8, 16, 32, 64};
   const int rows[3] = {2, 4, 8};
-  const int wgs[3] = {1, 2, 3};
+  const int wgs[4] = {0, 1, 2, 3};
   auto consider = [&](const culsh_schedule& s) -> int {
     if (same_schedule(s, best)) return CULSH_OK;
     float t = 0.f;
@@ -488,11 +502,13 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
   for (int v : wgs) {
     culsh_schedule s = best;
     s.wgs_per_cu = v;
+    if (s.fused && v == 0) continue;
     TRY(consider(s));
   }
   for (int f = 0; f < 2; ++f) {
     culsh_schedule s = best;
     s.fused = f;
+    if (f && s.wgs_per_cu == 0) s.wgs_per_cu = 3;
     TRY(consider(s));
   }
   TRY(culsh_bt_set_schedule(bt, &best));

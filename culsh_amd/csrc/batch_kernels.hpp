@@ -79,7 +79,7 @@ struct BatchArgs {
   unsigned long long* cu_q;
   int* ovl;
   unsigned* lhist;  // [kLocBins] locality histogram, then bin offsets
-  unsigned* lkey;   // [Qtot] locality keys
+  unsigned* lkey;   // [Qtot] coarse-cell keys (probe stage), then sort keys
   int* order;       // [Qtot] candidate-stage order (locality within class)
   // Index.
   const float* X;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
     const int te = active ? t : 0;
     probe8_pair<J, MI>(y, active, te, D, M, C.T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb,
                        a.shift, a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * Ptot + poff,
-                       a.cnu_qt + (long long)i * L + te);
+                       a.cnu_qt + (long long)i * L + te, te == 0 ? a.lkey + i : nullptr);
   }
 }
 
@@ -289,21 +289,13 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
 constexpr int kLocBins = 1 << 16;
 constexpr int kLocChunk = 16;  // consecutive ordered queries per XCD claim
 
+// Sort key of batch index i: its class, then the 12-bit coarse-cell key the
+// probe stage left in lkey[i] (table 0).
 __device__ inline unsigned batch_loc_key(const BatchArgs& a, const BatchHdr* hp, int i) {
   const int nclass = hp->nclass;
   int c = 0;
   while (c + 1 < nclass && a.classes[c + 1].b0 <= i) ++c;
-  const float* y = hp->Qm + a.bq[i].qrow * a.D;
-  const int mm = a.M < 6 ? a.M : 6;
-  uint64_t cell = 0;
-  for (int m = 0; m < mm; ++m) {
-    float s = 0.f;
-    for (int d = 0; d < a.D; ++d) s = fmaf(y[d], a.Ad[(long long)d * a.Jp + m], s);
-    const int cm = (int)floorf((s - (float)a.b64[m]) / (4.f * (float)a.W));
-    cell = cell * 0x100000001B3ull + (uint64_t)(uint32_t)cm;
-  }
-  const unsigned b = (unsigned)(mix64(cell) >> 52);
-  return ((unsigned)min(c, 15) << 12) | b;
+  return ((unsigned)min(c, 15) << 12) | (a.lkey[i] & 0xFFFu);
 }
 
 __global__ __launch_bounds__(256) void k_batch_lzero(BatchArgs a) {
@@ -315,7 +307,7 @@ __global__ __launch_bounds__(256) void k_batch_lhist(BatchArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= a.hdr->Qtot) return;
   const unsigned k = batch_loc_key(a, a.hdr, i);
-  a.lkey[i] = k;
+  a.lkey[i] = k;  // the full sort key, for the scatter
   atomicAdd(&a.lhist[k], 1u);
 }
 
@@ -410,6 +402,46 @@ __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
                              &hp->n_recheck, B)) {
       if (tid == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
     }
+  }
+}
+
+// Split schedule, candidate stage without persistence: one workgroup per
+// query (the grid covers the plan's capacity; workgroups past the batch
+// exit).  Workgroup b runs on XCD b % 8 and takes the (b / 8)-th position of
+// that XCD's chunks of the locality order, so dispatch follows the order
+// (heaviest classes first) and neighbouring queries share an L2.
+template <int R>
+__global__ __launch_bounds__(256) void k_batch_cand_each(BatchArgs a) {
+  __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
+  BatchHdr* hp = a.hdr;
+  const int Qtot = hp->Qtot;
+  const long long k = blockIdx.x >> 3, x = blockIdx.x & 7;
+  const long long p = ((k / kLocChunk) * 8 + x) * kLocChunk + k % kLocChunk;
+  if (p >= Qtot) return;
+  const int i = a.order[p];
+  CandBufs B;
+  B.hash = s_pool;
+  B.hbits = 13;
+  B.ulist = s_pool + kHcap;
+  B.ucap = kUcap;
+  B.d32 = reinterpret_cast<float*>(s_pool);
+  B.sd = reinterpret_cast<double*>(s_pool + kUcap);
+  B.sid = s_pool + kUcap + 2 * kScap;
+  B.scap = kScap;
+  B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  const BatchQuery d = a.bq[i];
+  QueryIO io;
+  io.y = hp->Qm + d.qrow * a.D;
+  io.qbq = a.qb + d.poff;
+  io.tstride = hp->Ptot;
+  io.T = d.T;
+  io.K = d.K;
+  io.oid = hp->out_ids + d.out;
+  io.od = hp->out_d + d.out;
+  io.slot = i;
+  if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+                           &hp->n_recheck, B)) {
+    if (threadIdx.x == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
   }
 }
 

@@ -87,7 +87,8 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
 // product of f32 operands is exact, so widening on read is the oracle's
 // arithmetic.  orb is the octet's
 // probe8_octet_bytes(M) of LDS.  Writes qbp[p] (bucket of probe p, or -1)
-// for p < T and *cnu (summed sizes of the matched buckets) when active.
+// for p < T and *cnu (summed sizes of the matched buckets) when active, and
+// the query's 12-bit coarse-cell key to *cell_key when cell_key is given.
 // Every lane of the octet calls it; T is uniform over the wave.
 template <int J, int MI>
 __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
@@ -97,7 +98,8 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
                                    const uint64_t* __restrict__ dir,
                                    const BucketRec* __restrict__ recs,
                                    const int16_t* __restrict__ bcodes, unsigned char* orb,
-                                   int* __restrict__ qbp, unsigned long long* __restrict__ cnu) {
+                                   int* __restrict__ qbp, unsigned long long* __restrict__ cnu,
+                                   unsigned* __restrict__ cell_key = nullptr) {
   const int lane8 = threadIdx.x & 7;
   double* ds = reinterpret_cast<double*>(orb);
   signed char* perm = reinterpret_cast<signed char*>(ds + 2 * M);
@@ -136,6 +138,18 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     }
   }
   const uint64_t basesum = octet_sum_u64(term);
+  if (cell_key) {
+    // Coarse cell of the query: floor(v_m / 4) of the first min(6, M)
+    // hashes, folded into 12 bits (a locality key; results never depend on it).
+    uint64_t ct = 0;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int m = lane8 + 8 * i;
+      if (valid[i] && m < 6) ct += mix64(((uint64_t)m << 32) ^ (uint32_t)(int)floor(v[i] * 0.25));
+    }
+    ct = octet_sum_u64(ct);
+    if (active && lane8 == 0) *cell_key = (unsigned)(mix64(ct) >> 52);
+  }
   const int maxT = (2 * M < 64) ? 2 * M : 64;
 
   if (T > 1) {

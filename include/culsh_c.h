@@ -165,7 +165,9 @@ typedef struct {
   int queries_per_item;/* probe-group width: queries per work item (a multiple
                           of probe_queries, <= 256) */
   int rows_in_flight;  /* candidate chunk: rows per 16-lane group (2, 4, 8) */
-  int wgs_per_cu;      /* persistent workgroups per compute unit (1..3) */
+  int wgs_per_cu;      /* persistent workgroups per compute unit (1..3); 0: the
+                          split schedule's candidate stage runs one workgroup
+                          per query instead */
   int fused;           /* 1: one persistent kernel, a work item = a block of
                           queries (probes over all tables, then candidates);
                           0: a probe kernel over every request, then a

@@ -97,6 +97,8 @@ def test_batch_matches_alone_and_oracle():
 SCHEDULES = [
     dict(probe_queries=32, probe_tables=1, queries_per_item=32, rows_in_flight=4, wgs_per_cu=3,
          fused=0),
+    dict(probe_queries=16, probe_tables=2, queries_per_item=32, rows_in_flight=8, wgs_per_cu=0,
+         fused=0),
     dict(probe_queries=16, probe_tables=2, queries_per_item=16, rows_in_flight=2, wgs_per_cu=1,
          fused=0),
     dict(probe_queries=8, probe_tables=4, queries_per_item=64, rows_in_flight=8, wgs_per_cu=2,
@@ -197,6 +199,7 @@ def test_batch_rejects_large_T():
     assert not bt.search_device([(3, 0, 8, 0, 0)], dY, ids, d)
     assert not bt.set_schedule(probe_queries=4, probe_tables=8)
     assert not bt.set_schedule(probe_queries=16, probe_tables=2, queries_per_item=24)
+    assert not bt.set_schedule(wgs_per_cu=0, fused=1)
 
 
 def test_batch_autotune_keeps_results():
