@@ -533,6 +533,56 @@ __global__ __launch_bounds__(256) void k_candidates_global(
   }
 }
 
+// Fixed working sets of the asynchronous search's overflow kernel: workgroups
+// and the largest hash / id list of one query.
+constexpr int kOvAsyncGrid = 16;
+constexpr int kOvAsyncHbits = 21;
+constexpr unsigned kOvAsyncUcap = 1u << 20;
+
+// The overflowed queries of an asynchronous search: like k_candidates_global,
+// but the list length is read on the device and every workgroup owns a
+// fixed region of 2^hbits_max + 2 ucap_max words, so no host round trip
+// sizes it.  A query with more than ucap_max non-unique candidates, or more
+// FP64 rechecks than kScapG, sets *err (sticky: the host clears it).
+__global__ __launch_bounds__(256) void k_candidates_overflow(
+    const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
+    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
+    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
+    unsigned long long* __restrict__ cu_q, const int* __restrict__ list,
+    const int* __restrict__ n_list, uint32_t* __restrict__ scratch, int hbits_max,
+    unsigned ucap_max, unsigned long long* __restrict__ n_recheck, int* __restrict__ err) {
+  __shared__ double s_sd[kScapG];
+  __shared__ uint32_t s_sid[kScapG];
+  __shared__ float s_y[kYcap];
+  const int n = *n_list;
+  uint32_t* base = scratch + (size_t)blockIdx.x * ((1ull << hbits_max) + 2ull * ucap_max);
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int q = list[e];
+    const unsigned long long cnu = query_cnu(q, L, cnu_qt);
+    if (cnu > ucap_max) {
+      if (threadIdx.x == 0) atomicOr(err, 1);
+      continue;
+    }
+    int hbits = 13;
+    while ((1ull << hbits) < 2 * cnu) ++hbits;
+    CandBufs B;
+    B.hash = base;
+    B.hbits = hbits;
+    B.ulist = base + (1u << hbits_max);
+    B.ucap = ucap_max;
+    B.d32 = reinterpret_cast<float*>(base + (1u << hbits_max) + ucap_max);
+    B.sd = s_sd;
+    B.sid = s_sid;
+    B.scap = kScapG;
+    B.ys = s_y;
+    if (!query_candidates(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
+                          cnu_qt, cnu_q, cu_q, n_recheck, B)) {
+      if (threadIdx.x == 0) atomicOr(err, 1);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Precision form: grouped queries, norm-expansion distances on MFMA.
 // ---------------------------------------------------------------------------

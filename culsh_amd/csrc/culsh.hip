@@ -875,6 +875,13 @@ struct culsh_st {
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt, dmax;
   DevBuf<uint32_t> dscratch;  // working sets of overflowed queries
   DevBuf<int> dnov;
+  // Asynchronous device searches: fixed overflow working sets, a sticky
+  // device error flag, its pinned host copy and the completion event.
+  DevBuf<uint32_t> dovfix;
+  DevBuf<int> derr;
+  int* herr = nullptr;
+  hipEvent_t evdone = nullptr;
+  bool pending = false;
   // Precision form: locality order of the queries and group working sets.
   DevBuf<uint64_t> dlk0, dlk1;
   DevBuf<uint32_t> dlv0, dlv1, dlscr, dgscratch;
@@ -891,8 +898,11 @@ struct culsh_st {
   unsigned long long tot_u = 0, tot_nu = 0, rechecked = 0;
   culsh_times times{};
   ~culsh_st() {
+    if (pending && evdone) (void)hipEventSynchronize(evdone);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (evdone) (void)hipEventDestroy(evdone);
+    if (herr) (void)hipHostFree(herr);
     if (stream) (void)hipStreamDestroy(stream);
   }
   void clear_results() {
@@ -912,6 +922,39 @@ int st_setup(culsh_st* st) {
   if (!st->stream) HIP_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
   for (auto& e : st->ev)
     if (!e) HIP_TRY(hipEventCreate(&e));
+  if (!st->evdone) HIP_TRY(hipEventCreate(&st->evdone));
+  if (!st->herr) {
+    HIP_TRY(hipHostMalloc((void**)&st->herr, sizeof(int), hipHostMallocDefault));
+    *st->herr = 0;
+  }
+  if (!st->derr.p) {
+    TRY(st->derr.ensure(1));
+    HIP_TRY(hipMemset(st->derr.p, 0, sizeof(int)));
+  }
+  return CULSH_OK;
+}
+
+// Reports (once) an error of an earlier asynchronous search whose work has
+// completed; with `block`, first waits for the last one.
+int st_check_async(culsh_st* st, bool block) {
+  if (!st->pending) return CULSH_OK;
+  if (block) {
+    HIP_TRY(hipEventSynchronize(st->evdone));
+  } else {
+    const hipError_t q = hipEventQuery(st->evdone);
+    if (q == hipErrorNotReady) return CULSH_OK;
+    HIP_TRY(q);
+  }
+  st->pending = false;
+  if (*st->herr) {
+    *st->herr = 0;
+    HIP_TRY(hipMemset(st->derr.p, 0, sizeof(int)));
+    return fail(CULSH_E_LIMIT,
+                "an earlier asynchronous search had a query with more than %u non-unique "
+                "candidates or more than %d candidates within the FP32 error bound of its "
+                "K-th distance (its results were not written)",
+                kOvAsyncUcap, kScapG);
+  }
   return CULSH_OK;
 }
 
@@ -1034,7 +1077,12 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
 // global-memory kernel for queries that overflow it).  Synchronises to check
 // the per-query capacity flags.
 int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
-               hipStream_t s, bool timed) {
+               hipStream_t s, bool timed, bool wait = true) {
+  TRY(st_check_async(st, wait));
+  // The plain form can finish without the host: its overflowed queries go
+  // to a fixed-size device kernel.  The precision form sizes its group
+  // working sets on the host.
+  const bool async = !wait && st->form == CULSH_FORM_PLAIN;
   culsh_ht* ht = st->ht;
   const int L = ht->L, M = ht->M, D = ht->D, T = st->T, K = st->K;
   const long long QT = (long long)Q * T;
@@ -1128,6 +1176,21 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
   HIP_TRY(hipGetLastError());
+  if (async) {
+    TRY(st->dovfix.ensure((size_t)kOvAsyncGrid *
+                          ((1ull << kOvAsyncHbits) + 2ull * kOvAsyncUcap)));
+    hipLaunchKernelGGL(k_candidates_overflow, dim3(kOvAsyncGrid), dim3(256), 0, s, dX, dQ, Q, D,
+                       L, T, K, (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
+                       (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                       d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p,
+                       (const int*)st->dnov.p, st->dovfix.p, kOvAsyncHbits, kOvAsyncUcap,
+                       st->dcnt.p, st->derr.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(st->herr, st->derr.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(st->evdone, s));
+    st->pending = true;
+    return CULSH_OK;
+  }
   int nov = 0;
   HIP_TRY(hipMemcpyAsync(&nov, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1243,11 +1306,17 @@ int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_k
   if (!d_queries || !d_knn_ids || !d_knn_dists) return fail(CULSH_E_ARG, "null device pointer");
   if (!st->ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
   return run_search(st, d_queries, st->ht->dX, Q, d_knn_ids, d_knn_dists, (hipStream_t)stream,
-                    true);
+                    true, false);
+}
+
+int culsh_st_wait(culsh_st* st) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  return st_check_async(st, true);
 }
 
 int culsh_st_kernel_ms(culsh_st* st, float* probe_ms, float* candidates_ms) {
   if (!st || !st->ev[2]) return fail(CULSH_E_STATE, "no search has run");
+  TRY(st_check_async(st, true));
   float a = 0.f, b = 0.f;
   HIP_TRY(hipEventElapsedTime(&a, st->ev[0], st->ev[1]));
   HIP_TRY(hipEventElapsedTime(&b, st->ev[1], st->ev[2]));

@@ -198,6 +198,11 @@ class SearchTables(_Base):
                                                        ids.data_ptr(), dists.data_ptr(),
                                                        ctypes.c_void_p(stream)))
 
+    def wait(self):
+        """Waits for the last device search; False (and the message) if it or an
+        earlier asynchronous search failed."""
+        return self._ok(self._L.culsh_st_wait(self._h))
+
     def getKnnIds(self):
         p = self._L.culsh_st_knn_ids(self._h)
         return None if not p else _view(p, ctypes.c_int, self.Q * self.K).reshape(self.Q, self.K)

@@ -114,9 +114,17 @@ int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* data
 /* Device-resident search: queries, ids and dists are device pointers
  * ([D x Q], [K x Q], [K x Q]); runs on `stream` (a hipStream_t, 0 = null
  * stream) and returns without synchronising.  Query buckets and candidate
- * totals of this call are not copied to the host. */
+ * totals of this call are not copied to the host.  In the plain form the
+ * call never waits for the device: a query whose candidates exceed the
+ * device path's limits (2^20 non-unique candidates, 4096 candidates within
+ * the FP32 error bound of the K-th distance) is reported by culsh_st_wait,
+ * culsh_st_kernel_ms or a later call on the object that finds the search
+ * completed; its results are not written. */
 int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
                            float* d_knn_dists, void* stream);
+/* Waits for the last device search and reports its (or an earlier one's)
+ * failure. */
+int culsh_st_wait(culsh_st* st);
 /* Distance form of search(): CULSH_FORM_PLAIN (default) or _PRECISION. */
 int culsh_st_set_form(culsh_st* st, int form);
 

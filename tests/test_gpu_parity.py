@@ -226,6 +226,48 @@ def test_device_api_matches_host_api():
     np.testing.assert_array_equal(d.cpu().numpy(), g["dists"])
 
 
+def test_device_search_overflow_runs_on_the_device():
+    """The asynchronous device search handles queries beyond the LDS working
+    set with its fixed-size device kernel (no host round trip) and matches
+    the host API; a query beyond that kernel's 2^20 candidates is reported by
+    wait(), and the object keeps working."""
+    import torch
+    from culsh_amd import HashTables, SearchTables
+    print("device search overflow", flush=True)
+    X = wl.mixture(30000, 16, 3, 0.3, 81)
+    Y = wl.queries_near(X, 40, 0.05, 82)
+    A, b = wl.projection(3, 16, 3, 8.0, 83)
+    g = run_gpu(X, Y, A, b, 3, 3, 8.0, 10, 4, want_buckets=False)
+    assert int(g["cu"].max()) > 4096
+    st = g["st"]
+    dY = torch.from_numpy(Y).cuda()
+    ids = torch.empty((Y.shape[0], 10), dtype=torch.int32, device="cuda")
+    d = torch.empty((Y.shape[0], 10), dtype=torch.float32, device="cuda")
+    assert st.search_device(dY, ids, d), st.last_error
+    assert st.wait(), st.last_error
+    np.testing.assert_array_equal(ids.cpu().numpy(), g["ids"])
+    np.testing.assert_array_equal(d.cpu().numpy(), g["dists"])
+    # Every point in one bucket per table: 4 x 300,000 > 2^20 candidates.
+    rng = np.random.default_rng(7)
+    Xb = rng.standard_normal((300000, 4)).astype(np.float32)
+    Ab = np.zeros((4, 1, 4), np.float32)  # every code is exactly 0
+    bb = np.zeros((4, 1), np.float32)
+    ht = HashTables()
+    assert ht.reset(300000, 4, 4, 1, 1e6, Ab, bb)
+    assert ht.index(Xb)
+    st2 = SearchTables()
+    assert st2.reset(ht, 5, 1)
+    qy = torch.from_numpy(Xb[:2].copy()).cuda()
+    i2 = torch.empty((2, 5), dtype=torch.int32, device="cuda")
+    d2 = torch.empty((2, 5), dtype=torch.float32, device="cuda")
+    assert st2.search_device(qy, i2, d2)
+    assert not st2.wait()
+    assert "asynchronous" in st2.last_error
+    assert st2.wait()
+    assert st2.search(Xb[:2], 2, Xb), st2.last_error
+    assert (st2.getKnnIds()[:, 0] == np.arange(2)).all()
+
+
 def test_index_device_resident_dataset():
     import torch
     print("index from device", flush=True)
