@@ -215,10 +215,11 @@ int bt_capture(culsh_bt* bt) {
   const int grid = bt->ncu * s.wgs_per_cu;
   const int egrid = (int)grid_for(bt->Qcap);
   // Split schedule: one workgroup per probe item, as many as a batch within
-  // the plan's capacities can have (every class adds at most one partial
-  // block).
+  // the plan's capacities can have (every class -- at most one per T <= 64 --
+  // adds at most one partial block).
   const long long pcap = ((long long)(bt->Qcap + s.probe_queries - 1) / s.probe_queries +
-                          bt->Rcap) * ((ht->L + s.probe_tables - 1) / s.probe_tables);
+                          std::min(bt->Rcap, 64)) *
+                         ((ht->L + s.probe_tables - 1) / s.probe_tables);
   if (pcap >= (1ll << 31)) return fail(CULSH_E_LIMIT, "batch probe grid too large");
   bt->drop_graph();
   HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));

@@ -284,10 +284,12 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
 // b_m) / 4W) of the first min(6, M) hashes of table 0) -- queries in one
 // coarse cell share most of their candidates, and the class (T) keeps heavy
 // queries first.  Grids are fixed (graph capture); bins beyond the batch stay
-// empty.  The order within a bin is whatever the atomics give: results do
-// not depend on the order.
+// empty; batches under kLocMinQ queries keep batch order (the sort would
+// cost more than it saves).  The order within a bin is whatever the atomics
+// give: results do not depend on the order.
 constexpr int kLocBins = 1 << 16;
-constexpr int kLocChunk = 16;  // consecutive ordered queries per XCD claim
+constexpr int kLocChunk = 16;     // consecutive ordered queries per XCD claim
+constexpr int kLocMinQ = 8192;    // smaller batches keep batch order
 
 // Sort key of batch index i: its class, then the 12-bit coarse-cell key the
 // probe stage left in lkey[i] (table 0).
@@ -299,13 +301,15 @@ __device__ inline unsigned batch_loc_key(const BatchArgs& a, const BatchHdr* hp,
 }
 
 __global__ __launch_bounds__(256) void k_batch_lzero(BatchArgs a) {
+  if (a.hdr->Qtot < kLocMinQ) return;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < kLocBins; i += gridDim.x * 256)
     a.lhist[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_batch_lhist(BatchArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.hdr->Qtot) return;
+  const int Qtot = a.hdr->Qtot;
+  if (i >= Qtot || Qtot < kLocMinQ) return;
   const unsigned k = batch_loc_key(a, a.hdr, i);
   a.lkey[i] = k;  // the full sort key, for the scatter
   atomicAdd(&a.lhist[k], 1u);
@@ -315,6 +319,7 @@ __global__ __launch_bounds__(256) void k_batch_lhist(BatchArgs a) {
 // consecutive bins per thread).
 __global__ __launch_bounds__(256) void k_batch_lscan(BatchArgs a) {
   __shared__ unsigned s_misc[4];
+  if (a.hdr->Qtot < kLocMinQ) return;
   const int tid = threadIdx.x;
   constexpr int per = kLocBins / 256;
   unsigned* h = a.lhist + tid * per;
@@ -331,7 +336,12 @@ __global__ __launch_bounds__(256) void k_batch_lscan(BatchArgs a) {
 
 __global__ __launch_bounds__(256) void k_batch_lscatter(BatchArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.hdr->Qtot) return;
+  const int Qtot = a.hdr->Qtot;
+  if (i >= Qtot) return;
+  if (Qtot < kLocMinQ) {
+    a.order[i] = i;  // small batch: batch order
+    return;
+  }
   const unsigned pos = atomicAdd(&a.lhist[a.lkey[i]], 1u);
   a.order[pos] = i;
 }

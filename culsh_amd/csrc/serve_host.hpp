@@ -246,7 +246,7 @@ int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_confi
     const int nslots = nindex * cfg->slots_per_index;
     const long long share = (long long)((double)freeb * cfg->hbm_fraction) / nslots;
     // Probe slots of all tables are addressed with 31-bit offsets.
-    long long Q = std::min<long long>(cfg->max_batch_queries > 0 ? cfg->max_batch_queries : 65536,
+    long long Q = std::min<long long>(cfg->max_batch_queries > 0 ? cfg->max_batch_queries : 4096,
                                       ((1ll << 31) - 1) / (32ll * maxL));
     while (Q > 1 && slot_bytes_for(maxL, Q, 32 * Q, (int)std::min<long long>(Q, 4096)) > share)
       Q /= 2;

@@ -227,7 +227,9 @@ typedef struct {
   int batching;           /* 1: continuous batching; 0: one request at a time
                              through the search path (the plain baseline) */
   double hbm_fraction;    /* share of free HBM the slots may use (0 < f <= 1) */
-  long long max_batch_queries; /* per batch (0 = 65536); HBM may lower it */
+  long long max_batch_queries; /* per batch (0 = 4096); HBM may lower it.  The
+                                  batch plans' fixed grids scale with it, so a
+                                  smaller cap means less per-batch overhead */
 } culsh_srv_config;
 
 typedef struct {
