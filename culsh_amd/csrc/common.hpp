@@ -74,9 +74,11 @@ constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
 
+// Every source lane of these patterns is inside the octet, so no lane takes
+// the bound_ctrl zero and the result needs no "old" operand (no copy).
 template <int CTRL>
 __device__ inline uint32_t dpp_u32(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
 // Reductions over the 8 lanes of an octet; every lane gets the result.  All
@@ -95,6 +97,51 @@ __device__ inline uint64_t octet_min_u64(uint64_t v) {
   CULSH_OCT_MIN64(kDppHalfMirror)
 #undef CULSH_OCT_MIN64
   return ((uint64_t)hi << 32) | lo;
+}
+
+// Lexicographic minimum of (v, slot) over the octet, carrying payload from
+// the winning lane; slots must be distinct within the octet.
+__device__ inline void octet_argmin_u64(uint64_t& v, uint32_t& slot, uint64_t& payload) {
+  uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v, ph = (uint32_t)(payload >> 32),
+           pl = (uint32_t)payload;
+#define CULSH_OCT_ARGMIN(C)                                                              \
+  {                                                                                      \
+    const uint32_t h2 = dpp_u32<C>(hi), l2 = dpp_u32<C>(lo), s2 = dpp_u32<C>(slot);      \
+    const uint32_t p2h = dpp_u32<C>(ph), p2l = dpp_u32<C>(pl);                           \
+    const uint64_t a = ((uint64_t)hi << 32) | lo, b = ((uint64_t)h2 << 32) | l2;         \
+    const bool lt = b < a || (b == a && s2 < slot);                                      \
+    hi = lt ? h2 : hi;                                                                   \
+    lo = lt ? l2 : lo;                                                                   \
+    slot = lt ? s2 : slot;                                                               \
+    ph = lt ? p2h : ph;                                                                  \
+    pl = lt ? p2l : pl;                                                                  \
+  }
+  CULSH_OCT_ARGMIN(kDppXor1)
+  CULSH_OCT_ARGMIN(kDppXor2)
+  CULSH_OCT_ARGMIN(kDppHalfMirror)
+#undef CULSH_OCT_ARGMIN
+  v = ((uint64_t)hi << 32) | lo;
+  payload = ((uint64_t)ph << 32) | pl;
+}
+
+__device__ inline void octet_argmin_u64(uint64_t& v, uint32_t& slot, uint32_t& payload) {
+  uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v;
+#define CULSH_OCT_ARGMIN(C)                                                              \
+  {                                                                                      \
+    const uint32_t h2 = dpp_u32<C>(hi), l2 = dpp_u32<C>(lo), s2 = dpp_u32<C>(slot);      \
+    const uint32_t p2 = dpp_u32<C>(payload);                                             \
+    const uint64_t a = ((uint64_t)hi << 32) | lo, b = ((uint64_t)h2 << 32) | l2;         \
+    const bool lt = b < a || (b == a && s2 < slot);                                      \
+    hi = lt ? h2 : hi;                                                                   \
+    lo = lt ? l2 : lo;                                                                   \
+    slot = lt ? s2 : slot;                                                               \
+    payload = lt ? p2 : payload;                                                         \
+  }
+  CULSH_OCT_ARGMIN(kDppXor1)
+  CULSH_OCT_ARGMIN(kDppXor2)
+  CULSH_OCT_ARGMIN(kDppHalfMirror)
+#undef CULSH_OCT_ARGMIN
+  v = ((uint64_t)hi << 32) | lo;
 }
 
 __device__ inline uint32_t octet_min_u32(uint32_t v) {

@@ -10,6 +10,8 @@
 #include "common.hpp"
 #include "delta_order.hpp"
 
+#include <type_traits>
+
 namespace culsh {
 
 // Per-octet LDS scratch (bytes): ds[2M] f64 (deltas, then sorted deltas),
@@ -166,10 +168,12 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
       }
     }
     wave_lds_sync();
-    // Rank of each delta = number of smaller deltas; the order is unique when
-    // no two deltas are equal.
+    // Rank of each delta = number of smaller deltas.  Equal deltas get equal
+    // ranks, so the order is unique exactly when no two deltas below rank
+    // maxT share a rank (equal deltas all at ranks >= maxT leave the first
+    // maxT unchanged): every delta claims perm[rank], and a claim that does
+    // not read back as its own marks a tie.
     int rdn[MI], rup[MI];
-    bool tie = false;
 #pragma unroll
     for (int i = 0; i < MI; ++i) rdn[i] = rup[i] = 0;
 #pragma unroll 4
@@ -177,26 +181,36 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
       const double dj = ds[j];
 #pragma unroll
       for (int i = 0; i < MI; ++i) {
-        const int m = lane8 + 8 * i;
         rdn[i] += dj < dn[i];
         rup[i] += dj < up[i];
-        tie |= valid[i] && ((dj == dn[i] && j != m) || (dj == up[i] && j != M + m));
       }
     }
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int m = lane8 + 8 * i;
+      if (valid[i] && rdn[i] < maxT) perm[rdn[i]] = (signed char)m;
+      if (valid[i] && rup[i] < maxT) perm[rup[i]] = (signed char)(M + m);
+    }
+    wave_lds_sync();
+    bool tie = false;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int m = lane8 + 8 * i;
+      tie |= valid[i] && rdn[i] < maxT && perm[rdn[i]] != (signed char)m;
+      tie |= valid[i] && rup[i] < maxT && perm[rup[i]] != (signed char)(M + m);
+    }
     const bool otie = octet_min_u32(tie ? 0u : 1u) == 0u;
-    wave_lds_sync();  // every lane has read ds before it is overwritten
+    wave_lds_sync();  // every lane has read ds and perm before they are rewritten
     if (!otie) {
 #pragma unroll
       for (int i = 0; i < MI; ++i) {
         const int m = lane8 + 8 * i;
         if (valid[i]) {
           if (rdn[i] < maxT) {
-            perm[rdn[i]] = (signed char)m;
             ds[rdn[i]] = dn[i];
             rank[m] = (unsigned char)rdn[i];
           }
           if (rup[i] < maxT) {
-            perm[rup[i]] = (signed char)(M + m);
             ds[rup[i]] = up[i];
             rank[M + m] = (unsigned char)rup[i];
           }
@@ -223,48 +237,50 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
 
   // Multiprobe queue (cuLSH_Kernels.cu:216-255).  Scores are non-negative
   // doubles, kept as their bit patterns (same order); empty slots hold ~0.
-  uint64_t qs[J], qc[J], pm[J];
+  // Sets as bit masks over the sorted deltas.  An entry made at step p has
+  // its highest bit at most p, so T <= 32 (J <= 4) keeps them in 32 bits.
+  using Code = typename std::conditional<(J <= 4), uint32_t, uint64_t>::type;
+  constexpr int kTop = 8 * (int)sizeof(Code) - 1;
+  uint64_t qs[J];
+  Code qc[J], pm[J];
 #pragma unroll
   for (int r = 0; r < J; ++r) {
     qs[r] = ~0ull;
-    qc[r] = 0ull;
-    pm[r] = 0ull;
+    qc[r] = 0;
+    pm[r] = 0;
   }
   if (T > 1) {
     if (lane8 == 0) {
-      qc[0] = 1ull;
+      qc[0] = 1;
       qs[0] = (uint64_t)__double_as_longlong(ds[0]);
     }
     int size = 1;
     for (int p = 1; p < T; ++p) {
-      uint64_t best = ~0ull;
+      // Pop: the lowest slot (s = 8 r + lane) among the minimal scores.
+      uint64_t omin = ~0ull;
+      Code probe = 0;
       int br = 0;
 #pragma unroll
       for (int r = 0; r < J; ++r)
-        if (qs[r] < best) {
-          best = qs[r];
+        if (qs[r] < omin) {
+          omin = qs[r];
           br = r;
+          probe = qc[r];
         }
-      const uint64_t omin = octet_min_u64(best);
-      const uint32_t myslot = best == omin ? (uint32_t)(br * 8 + lane8) : 0xFFFFFFFFu;
-      const uint32_t ws = octet_min_u32(myslot);
+      uint32_t ws = (uint32_t)(br * 8 + lane8);
+      octet_argmin_u64(omin, ws, probe);
       const int wl = (int)(ws & 7), wr = (int)(ws >> 3);
-      uint64_t sel = 0ull;
-#pragma unroll
-      for (int r = 0; r < J; ++r)
-        if (r == wr) sel = qc[r];
-      const uint64_t probe = octet_or_u64(lane8 == wl ? sel : 0ull);
       if (lane8 == (p & 7)) {
 #pragma unroll
         for (int r = 0; r < J; ++r)
           if (r == (p >> 3)) pm[r] = probe;
       }
-      const int maxPos = 63 - __clzll((long long)probe);
+      const int maxPos = kTop - (J <= 4 ? __clz((int)probe) : __clzll((long long)probe));
       if (maxPos < maxT - 1) {
         const double e = __longlong_as_double((long long)omin) + ds[maxPos + 1];
-        const uint64_t pe = probe | (1ull << (maxPos + 1));
+        const Code pe = probe | ((Code)1 << (maxPos + 1));
         const double sh = e - ds[maxPos];
-        const uint64_t ps = pe - (1ull << maxPos);
+        const Code ps = pe - ((Code)1 << maxPos);
         if (lane8 == wl) {
 #pragma unroll
           for (int r = 0; r < J; ++r)

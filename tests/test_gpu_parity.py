@@ -140,6 +140,24 @@ def test_exact_delta_ties():
     assert_search_equal(g, o)
 
 
+def test_delta_ties_around_the_probe_window():
+    """M=40: 80 deltas, of which the first 64 feed the queue.  Duplicated
+    columns tie a dimension's "-1" and "+1" deltas with another's; one of the
+    two tied pairs usually ranks inside the first 64 and the other often
+    beyond, so both the heap-order path and the plain sort are exercised."""
+    print("delta ties M=40", flush=True)
+    X = wl.mixture(4000, 16, 8, 0.3, 41)
+    Y = wl.queries_near(X, 80, 0.05, 42)
+    L, M, W = 2, 40, 1.5
+    A, b = wl.projection(L, 16, M, W, 43)
+    for src, dst in ((1, 3), (0, 5), (7, 30), (12, 39), (20, 21)):
+        A[:, dst] = A[:, src]
+        b[:, dst] = b[:, src]
+    g = run_gpu(X, Y, A, b, L, M, W, 5, 64)
+    o = run_oracle(X, Y, A, b, L, M, W, 5, 64)
+    assert_search_equal(g, o)
+
+
 def test_config1_full():
     """c1: N=100k, D=128, L=8, M=16, W=4, Q=1k, K=10, T=1 -- ids exact."""
     cfg = wl.CONFIGS["c1"]
