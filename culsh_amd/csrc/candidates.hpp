@@ -26,24 +26,34 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr double kU32d = 0x1.0p-24;
 
 // FP64 squared L2 in the oracle's order (sequential FMA over d ascending).
-// Rows are fetched 32 floats at a time (8 independent 16-byte loads) ahead
-// of the dependent FMA chain when D is a multiple of 32.
+// When D is a multiple of 32 the row is fetched 32 floats at a time (8
+// independent 16-byte loads), the next 32 in flight while the current ones
+// go through the dependent FMA chain.
 __device__ inline double dist64(const float* __restrict__ q, const float* __restrict__ x, int D) {
   double acc = 0.0;
   if ((D & 31) == 0) {
     const float4* x4 = reinterpret_cast<const float4*>(x);
-    for (int d0 = 0; d0 < D; d0 += 32) {
-      float4 xv[8];
+    float4 cur[8], nxt[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) xv[k] = x4[(d0 >> 2) + k];
+    for (int k = 0; k < 8; ++k) cur[k] = x4[k];
+    for (int d0 = 0; d0 < D; d0 += 32) {
+      const bool more = d0 + 32 < D;
+      if (more) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nxt[k] = x4[((d0 + 32) >> 2) + k];
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const float* xs = reinterpret_cast<const float*>(&xv[k]);
+        const float* xs = reinterpret_cast<const float*>(&cur[k]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const double df = (double)q[d0 + 4 * k + e] - (double)xs[e];
           acc = fma(df, df, acc);
         }
+      }
+      if (more) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
       }
     }
     return acc;
@@ -117,49 +127,76 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
 }
 
 // Calls ins(id) for every member of every bucket a query matched
-// (cuLSH_SearchTables.cu:321-340): a wave takes 64 (table, probe) entries,
-// prefix-sums their bucket sizes and spreads the member ids over its lanes.
-// qbq[t * tstride + p] is the query's probe p of table t.  All threads of
-// the workgroup must call it.
+// (cuLSH_SearchTables.cu:321-340).  Wave w takes the (table, probe) entries
+// w*64 + lane + 256 i; the probe slots and bucket records of kE such rounds
+// are loaded together, then per round the wave prefix-sums the bucket sizes
+// and spreads the member ids over its lanes, kR rounds of 64 members with
+// their id loads in flight together.  qbq[t * tstride + p] is the query's
+// probe p of table t.  All threads of the workgroup must call it.
 template <class Ins>
 __device__ inline void for_each_member(const int* __restrict__ qbq, long long tstride, int L,
                                        int T, const BucketRec* __restrict__ recs,
                                        const uint32_t* __restrict__ sids, Ins ins) {
+  constexpr int kE = 8, kR = 8;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int LT = L * T;
-  for (int e0 = w * 64; e0 < LT; e0 += 256) {
-    const int e = e0 + lane;
-    uint32_t st = 0, sz = 0;
-    if (e < LT) {
-      const int t = e / T, p = e - t * T;
-      const int k = qbq[(long long)t * tstride + p];
-      if (k >= 0) {
-        st = recs[k].start;
-        sz = recs[k].size;
-      }
-    }
-    uint32_t inc = sz;
+  for (int p0 = w * 64; p0 < LT; p0 += 256 * kE) {
+    int kb[kE];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t yv = __shfl_up(inc, off);
-      if (lane >= off) inc += yv;
-    }
-    const uint32_t tot = __shfl(inc, 63);
-    for (uint32_t b = 0; b < tot; b += 64) {
-      const uint32_t i = b + lane;
-      int lo = 0, hi = 63;
-      while (lo < hi) {  // first lane whose inclusive prefix exceeds i
-        const int mid = (lo + hi) >> 1;
-        const uint32_t im = __shfl(inc, mid);
-        if (im > i)
-          hi = mid;
-        else
-          lo = mid + 1;
+    for (int i = 0; i < kE; ++i) {
+      const int e = p0 + 256 * i + lane;
+      kb[i] = -1;
+      if (e < LT) {
+        const int t = e / T, p = e - t * T;
+        kb[i] = qbq[(long long)t * tstride + p];
       }
-      const uint32_t st_l = __shfl(st, lo);
-      const uint32_t inc_l = __shfl(inc, lo);
-      const uint32_t sz_l = __shfl(sz, lo);
-      if (i < tot) ins(sids[st_l + (i - (inc_l - sz_l))]);
+    }
+    uint32_t st[kE], sz[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      st[i] = 0;
+      sz[i] = 0;
+      if (kb[i] >= 0) {
+        st[i] = recs[kb[i]].start;
+        sz[i] = recs[kb[i]].size;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      if (p0 + 256 * i >= LT) break;  // uniform over the wave
+      uint32_t inc = sz[i];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t yv = __shfl_up(inc, off);
+        if (lane >= off) inc += yv;
+      }
+      const uint32_t tot = __shfl(inc, 63);
+      for (uint32_t b = 0; b < tot; b += 64 * kR) {
+        uint32_t id[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          id[r] = kEmpty;
+          if (b + 64 * r < tot) {  // uniform over the wave
+            const uint32_t m = b + 64 * r + lane;
+            int lo = 0, hi = 63;
+            while (lo < hi) {  // first lane whose inclusive prefix exceeds m
+              const int mid = (lo + hi) >> 1;
+              const uint32_t im = __shfl(inc, mid);
+              if (im > m)
+                hi = mid;
+              else
+                lo = mid + 1;
+            }
+            const uint32_t st_l = __shfl(st[i], lo);
+            const uint32_t inc_l = __shfl(inc, lo);
+            const uint32_t sz_l = __shfl(sz[i], lo);
+            if (m < tot) id[r] = sids[st_l + (m - (inc_l - sz_l))];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r)
+          if (id[r] != kEmpty) ins(id[r]);
+      }
     }
   }
 }
@@ -228,22 +265,23 @@ __device__ inline bool finish_topk(const float* __restrict__ X, const float* y, 
     __syncthreads();
   }
 
-  // Candidates that can be at or below the true K-th distance, in FP64.
+  // Candidates that can be at or below the true K-th distance: their ids
+  // first, then their FP64 distances, one row per thread (so a wave's rows
+  // are re-read together rather than one chain per qualifying lane).
   for (int u = tid; u < U; u += 256) {
     if ((double)fmaxf(dapx[u], 0.f) <= thr) {
       const unsigned i = atomicAdd(&s_scount, 1u);
-      if (i < scap) {
-        const uint32_t id = ids[u];
-        sd[i] = dist64(y, X + (long long)id * D, D);
-        sid[i] = id;
-      } else {
+      if (i < scap)
+        sid[i] = ids[u];
+      else
         s_ovf = 1;
-      }
     }
   }
   __syncthreads();
   if (s_ovf) return false;
   const int S = (int)s_scount;
+  for (int i = tid; i < S; i += 256) sd[i] = dist64(y, X + (long long)sid[i] * D, D);
+  __syncthreads();
   if (tid == 0 && S > 0) atomicAdd(n_recheck, (unsigned long long)S);
   // Candidates at or beyond FLT_MAX are never admitted by the reference's
   // heap (initialised to FLT_MAX, strict <, cuLSH_Kernels.cu:426).
