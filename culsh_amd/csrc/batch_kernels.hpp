@@ -371,7 +371,7 @@ __device__ inline int claim_cand(const BatchArgs& a, BatchHdr* hp, int Qtot, int
 // Split schedule, candidate stage: one query per work item, claimed from the
 // sharded queue in locality order (heaviest classes first).
 template <int R>
-__global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
+__global__ __launch_bounds__(256, 3) void k_batch_cand(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
   __shared__ unsigned s_item;
   const int tid = threadIdx.x;
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void k_batch_cand(BatchArgs a) {
 // that XCD's chunks of the locality order, so dispatch follows the order
 // (heaviest classes first) and neighbouring queries share an L2.
 template <int R>
-__global__ __launch_bounds__(256) void k_batch_cand_each(BatchArgs a) {
+__global__ __launch_bounds__(256, 3) void k_batch_cand_each(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_pool[kHcap + kUcap];
   BatchHdr* hp = a.hdr;
   const int Qtot = hp->Qtot;

@@ -488,13 +488,14 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   return true;
 }
 
-// One workgroup per query, all buffers in LDS (~50 KB).  After the union the
+// One workgroup per query, all buffers in LDS (~50 KB, so three workgroups
+// per CU; the launch bound holds the registers to that).  After the union the
 // hash slots hold d32 [kUcap], sd [kScap] (f64), sid [kScap] and the query.
 // Queries that do not fit are appended to overflow[] for
 // k_candidates_global.  With `order` (a locality order of the queries),
 // XCD x (which receives workgroups x, x+8, ...) takes the x-th eighth of the
 // order in sequence, so queries sharing candidate rows run on one L2.
-__global__ __launch_bounds__(256) void k_candidates(
+__global__ __launch_bounds__(256, 3) void k_candidates(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
     const int* __restrict__ qb, const BucketRec* __restrict__ recs,
     const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
@@ -681,7 +682,7 @@ __global__ __launch_bounds__(256) void k_group_cnu(const uint32_t* __restrict__ 
 // |d~ - d| <= 2 (D+4) u (|q| + xmax)^2 (Higham ch. 3, margin 2).  Then each
 // query runs finish_topk on its own candidates.  Queries that need more
 // FP64 rechecks than the LDS holds go to overflow[].
-__global__ __launch_bounds__(256) void k_group_candidates(
+__global__ __launch_bounds__(256, 3) void k_group_candidates(
     const float* __restrict__ X, const float* __restrict__ xn2, float xmax,
     const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
     const int* __restrict__ qb, const BucketRec* __restrict__ recs,
