@@ -228,9 +228,11 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
 
 // Split schedule, probe stage: probe items (class, block of qn queries, tn
 // tables) -- 32 (query, table) pairs, one pass -- one per workgroup, items
-// of heavier classes first.  (A grid-stride loop over items keeps the
-// projection's values live across iterations and doubles the registers.)  Needs batch_probe_lds(M) bytes of dynamic LDS.
-// Query vectors are read through L1, as in k_query_probe8.
+// of heavier classes first and, within a class, table groups in sequence
+// (the workgroups in flight look up few tables, as in k_query_probe8).  (A
+// grid-stride loop over items keeps the projection's values live across
+// iterations and doubles the registers.)  Needs batch_probe_lds(M) bytes of
+// dynamic LDS.  Query vectors are read through L1, as in k_query_probe8.
 __host__ __device__ inline int batch_probe_lds(int M) { return 32 * probe8_octet_bytes(M); }
 
 // One launch per queue width J (index ji: J = 1 << ji) over that width's
@@ -251,7 +253,6 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
   const int nclass = hp->nclass, qn = hp->qn, tn = hp->tn;
   const int it_lo = hp->pj_lo[ji], it_hi = hp->pj_hi[ji];
   const int D = a.D, M = a.M, L = a.L;
-  const int ntg = (L + tn - 1) / tn;
   unsigned char* orb = smem + oct * probe8_octet_bytes(M);
   const int qi = oct % qn, ti = oct / qn;
   {
@@ -261,7 +262,8 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
     while (c + 1 < nclass && a.classes[c + 1].pitem0 <= it) ++c;
     const BatchClass C = a.classes[c];
     const int local = it - C.pitem0;
-    const int bl = local / ntg, g = local - bl * ntg;
+    const int nbl = (C.n + qn - 1) / qn;  // the class's items: table groups in sequence
+    const int g = local / nbl, bl = local - g * nbl;
     const int i0 = C.b0 + bl * qn;
     const int ns = min(qn, C.b0 + C.n - i0);
     if (tid < qn) s_qrow[tid] = tid < ns ? a.bq[i0 + tid].qrow : 0;

@@ -332,9 +332,11 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
   if (active && lane8 == 0) *cnu = sizesum;
 }
 
-// The search's octet kernel: 32 queries of one table per workgroup.
-// qb[t][q*T + p], cnu_qt[q*L + t].  Needs probe8_lds_bytes(D, M) bytes of
-// dynamic LDS.
+// The search's octet kernel: 32 queries of one table per workgroup, tables
+// in sequence (workgroup b: table b / ceil(Q/32)), so the workgroups in
+// flight look up one or two tables' directory, records and codes at a time
+// and those stay in cache.  qb[t][q*T + p], cnu_qt[q*L + t].  Needs
+// probe8_lds_bytes(D, M) bytes of dynamic LDS.
 template <int J, int MI>
 __global__ __launch_bounds__(256) void k_query_probe8(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
@@ -344,8 +346,9 @@ __global__ __launch_bounds__(256) void k_query_probe8(
     int* __restrict__ qb, unsigned long long* __restrict__ cnu_qt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, oct = tid >> 3;
-  const int t = blockIdx.x % L;
-  const int q0 = (blockIdx.x / L) * 32;
+  const int nqb = (Q + 31) / 32;
+  const int t = (int)blockIdx.x / nqb;
+  const int q0 = ((int)blockIdx.x - t * nqb) * 32;
   const int q = q0 + oct;
   const bool active = q < Q;
   unsigned char* orb = smem + oct * probe8_octet_bytes(M);
