@@ -105,8 +105,10 @@ BatchKernel batch_probe_kernel_j(int M) {
   return k_batch_probe<8, J>;
 }
 
-// Queue width index of a probe count: J = 1, 2, 4, 8 slots per lane.
-int batch_ji(int T) { return T <= 8 ? 0 : T <= 16 ? 1 : T <= 32 ? 2 : 3; }
+// Queue width index of a probe count (J = 1 << index slots per lane): the
+// split schedule runs T <= 32 at J = 4 (the narrower widths take as many
+// registers) and T <= 64 at J = 8.
+int batch_ji(int T) { return T <= 32 ? 2 : 3; }
 
 BatchKernel batch_cand_kernel(int R) {
   if (R == 2) return k_batch_cand<2>;
@@ -231,13 +233,11 @@ int bt_capture(culsh_bt* bt) {
     hipLaunchKernelGGL(batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu), dim3(grid),
                        dim3(256), 0, bt->cap, a);
   } else {
-    // Widest queues first; a launch whose range is empty exits at once.
+    // Wide queues first; a launch whose range is empty exits at once.
     const int plds = batch_probe_lds(ht->M);
     const dim3 pg((unsigned)pcap);
     hipLaunchKernelGGL(batch_probe_kernel_j<8>(ht->M), pg, dim3(256), plds, bt->cap, a);
     hipLaunchKernelGGL(batch_probe_kernel_j<4>(ht->M), pg, dim3(256), plds, bt->cap, a);
-    hipLaunchKernelGGL(batch_probe_kernel_j<2>(ht->M), pg, dim3(256), plds, bt->cap, a);
-    hipLaunchKernelGGL(batch_probe_kernel_j<1>(ht->M), pg, dim3(256), plds, bt->cap, a);
     // Locality order of the candidate stage (counting sort, fixed grids).
     hipLaunchKernelGGL(k_batch_lzero, dim3(64), dim3(256), 0, bt->cap, a);
     hipLaunchKernelGGL(k_batch_lhist, dim3(egrid), dim3(256), 0, bt->cap, a);

@@ -51,7 +51,7 @@ struct BatchHdr {
   long long Ptot;   // probe slots per table
   int nreq, nclass, Qtot, nitems;  // nitems: fused work items
   int nprobe;                      // split-schedule probe items
-  int pj_lo[4], pj_hi[4];          // their ranges per queue width J = 1, 2, 4, 8
+  int pj_lo[4], pj_hi[4];          // their ranges per queue width J = 1 << index
   int qn, tn, bq;  // schedule: probe tile qn x tn, queries per fused block
   // Device-written counters (zero in the host copy).
   unsigned next_item;      // fused schedule's work queue
@@ -235,8 +235,9 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
 // dynamic LDS.  Query vectors are read through L1, as in k_query_probe8.
 __host__ __device__ inline int batch_probe_lds(int M) { return 32 * probe8_octet_bytes(M); }
 
-// One launch per queue width J (index ji: J = 1 << ji) over that width's
-// contiguous item range, so each launch has the registers of its own J; the
+// One launch per queue width J (index ji: J = 1 << ji; the host uses J = 4
+// and 8) over that width's contiguous item range, so each launch has the
+// registers of its own J; the
 // grid covers the plan's largest possible range and the workgroups past
 // this batch's range exit at once.
 template <int MI, int J>
