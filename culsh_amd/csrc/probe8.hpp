@@ -33,6 +33,20 @@ __host__ __device__ inline int probe8_lds_bytes(int D, int M) {
   return 32 * probe8_octet_bytes(M);
 }
 
+// Bucket of probe key `key` with directory entry e, for a caller that then
+// verifies the bucket's full code: a single-entry slot whose fingerprint
+// matches needs no key load (the slot's keys are all of this table, and
+// equal codes of one table have equal keys); larger slots are searched by
+// key.  -1 when absent.
+__device__ inline long long lookup_unverified(uint64_t key, uint64_t e,
+                                              const uint64_t* __restrict__ dir, int shift,
+                                              const BucketRec* __restrict__ recs) {
+  const uint32_t cnt = (uint32_t)(e >> 56);
+  if (cnt == 0) return -1;
+  if (cnt == 1) return ((e >> 32) & kFpMask) == (key & kFpMask) ? (long long)(uint32_t)e : -1;
+  return lookup_resolve(key, e, dir, shift, recs);
+}
+
 // Does the probe code of `mask` (base code plus the mask's +-1 moves) equal
 // the bucket code bc?  Compares the dimensions where the bucket differs from
 // the base code with the dimensions the mask moves.
@@ -322,10 +336,15 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
   for (int r = 0; r < J; ++r) {
     const int p = r * 8 + lane8;
     if (active && p < T) {
-      long long k = lookup_resolve(key[r], ent[r], dir, shift, recs);
-      if (k >= 0 && !probe_matches(bcodes + k * M, base, perm, rank, M, pm[r])) k = -1;
+      long long k = lookup_unverified(key[r], ent[r], dir, shift, recs);
+      if (k >= 0) {
+        const uint32_t size = recs[k].size;  // in flight with the code loads
+        if (probe_matches(bcodes + k * M, base, perm, rank, M, pm[r]))
+          sizesum += size;
+        else
+          k = -1;
+      }
       qbp[p] = (int)k;
-      if (k >= 0) sizesum += recs[k].size;
     }
   }
   sizesum = octet_sum_u64(sizesum);
