@@ -430,20 +430,30 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   const int tid = threadIdx.x;
   const float* y = io.y;
   const int K = io.K;
-  const uint32_t hslots = 1u << B.hbits, hmask = hslots - 1;
 
   __syncthreads();  // a previous query's readers of the shared counters are done
-  if (tid == 0) {
-    s_cnu = query_cnu((int)io.slot, L, cnu_qt);
-    s_count = 0;
-    s_ovf = 0;
+  if (tid < 64) {
+    unsigned long long c = 0;
+    for (int t = tid; t < L; t += 64) c += cnu_qt[io.slot * L + t];
+    c = wave_sum_u64(c);
+    if (tid == 0) {
+      s_cnu = c;
+      s_count = 0;
+      s_ovf = 0;
+    }
   }
-  for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
   __syncthreads();
   const unsigned long long cnu = s_cnu;
+  // Hash slots: 2^hbits >= 2 C_nu (at least 256, at most the buffer's), so a
+  // light query clears only what it uses.
+  int hbits = 8;
+  while (hbits < B.hbits && (1ull << hbits) < 2 * cnu) ++hbits;
+  const uint32_t hslots = 1u << hbits, hmask = hslots - 1;
+  for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
+  __syncthreads();
   if (cnu > 0) {
     for_each_member(io.qbq, io.tstride, L, io.T, recs, sids, [&](uint32_t id) {
-      uint32_t h = (id * 0x9E3779B1u) >> (32 - B.hbits);
+      uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
       for (uint32_t probe = 0;; ++probe) {
         if (probe == hslots) {
           s_ovf = 1;
