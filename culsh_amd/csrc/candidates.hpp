@@ -73,7 +73,55 @@ template <int R = 4>
 __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __restrict__ q,
                                     int D, const uint32_t* ids, int U, float* d32) {
   const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
-  if ((D & 3) == 0) {
+  if ((D & 31) == 0 && D <= 96) {
+    // D = 32, 64, 96: 8 lanes per row with D/32 16-byte loads each (no idle
+    // lanes), R8 rows per group, so a workgroup has 32 R8 rows in flight.
+    // Any summation order keeps the error bound the caller uses.
+    constexpr int R8 = R >= 8 ? 6 : R;
+    const int g8 = threadIdx.x >> 3, l8 = threadIdx.x & 7;
+    const int nf = D >> 5;
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    float4 qv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qv[k] = k < nf ? q4[l8 + 8 * k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int u0 = g8 * R8; u0 < U; u0 += 32 * R8) {
+      float4 xv[R8][3];
+#pragma unroll
+      for (int c = 0; c < R8; ++c) {
+        const float4* xr =
+            reinterpret_cast<const float4*>(X + (long long)ids[min(u0 + c, U - 1)] * D);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          xv[c][k] = k < nf ? xr[l8 + 8 * k] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      float acc[R8];
+#pragma unroll
+      for (int c = 0; c < R8; ++c) {
+        acc[c] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k < nf) {
+            const float d0 = qv[k].x - xv[c][k].x, d1 = qv[k].y - xv[c][k].y,
+                        d2 = qv[k].z - xv[c][k].z, d3 = qv[k].w - xv[c][k].w;
+            acc[c] = fmaf(d0, d0, acc[c]);
+            acc[c] = fmaf(d1, d1, acc[c]);
+            acc[c] = fmaf(d2, d2, acc[c]);
+            acc[c] = fmaf(d3, d3, acc[c]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < R8; ++c) {
+#pragma unroll
+        for (int off = 4; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 8);
+      }
+      if (l8 == 0) {
+#pragma unroll
+        for (int c = 0; c < R8; ++c)
+          if (u0 + c < U) d32[u0 + c] = acc[c];
+      }
+    }
+  } else if ((D & 3) == 0) {
     const int D4 = D >> 2;
     const float4* q4 = reinterpret_cast<const float4*>(q);
     for (int u0 = g * R; u0 < U; u0 += 16 * R) {
