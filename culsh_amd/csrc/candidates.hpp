@@ -81,6 +81,9 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
     const int g8 = threadIdx.x >> 3, l8 = threadIdx.x & 7;
     const int nf = D >> 5;
     const float4* q4 = reinterpret_cast<const float4*>(q);
+    float4 qv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qv[k] = k < nf ? q4[l8 + 8 * k] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int u0 = g8 * R8; u0 < U; u0 += 32 * R8) {
       float4 xv[R8][3];
 #pragma unroll
@@ -98,9 +101,8 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           if (k < nf) {
-            const float4 qk = q4[l8 + 8 * k];  // staged query: LDS when the caller staged it
-            const float d0 = qk.x - xv[c][k].x, d1 = qk.y - xv[c][k].y,
-                        d2 = qk.z - xv[c][k].z, d3 = qk.w - xv[c][k].w;
+            const float d0 = qv[k].x - xv[c][k].x, d1 = qv[k].y - xv[c][k].y,
+                        d2 = qv[k].z - xv[c][k].z, d3 = qv[k].w - xv[c][k].w;
             acc[c] = fmaf(d0, d0, acc[c]);
             acc[c] = fmaf(d1, d1, acc[c]);
             acc[c] = fmaf(d2, d2, acc[c]);
