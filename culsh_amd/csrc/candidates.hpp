@@ -67,8 +67,9 @@ __device__ inline double dist64(const float* __restrict__ q, const float* __rest
 }
 
 // Direct FP32 squared L2 of the U candidate rows ids[0..U) against q
-// (cuLSH_Kernels.cu:319-346): 16 lanes per row, R rows per 16-lane group
-// with their loads issued together (the candidate chunk).  Writes d32[u].
+// (cuLSH_Kernels.cu:319-346): a lane group per row (8 lanes when D is 32, 64
+// or 96, else 16), R rows per group with their loads issued together (the
+// candidate chunk).  Writes d32[u].
 template <int R = 4>
 __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __restrict__ q,
                                     int D, const uint32_t* ids, int U, float* d32) {

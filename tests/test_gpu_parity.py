@@ -30,6 +30,20 @@ def test_small_mixture(T, K):
     assert_search_equal(g, o)
 
 
+@pytest.mark.parametrize("D", [32, 64, 96])
+def test_row_widths_of_the_8_lane_distance_path(D):
+    """D = 32, 64, 96 take the candidate stage's 8-lanes-per-row FP32
+    distances (1, 2, 3 loads per lane); a few hundred candidates per query
+    (wide W) so every row round and the near-K-th FP64 recheck run."""
+    print("8-lane rows D=%d" % D, flush=True)
+    X, Y, A, b = _small(900 + D, N=6000, D=D, L=4, M=6, W=3.0, Q=60, centers=12, sigma=0.2)
+    g = run_gpu(X, Y, A, b, 4, 6, 3.0, 20, 8)
+    o = run_oracle(X, Y, A, b, 4, 6, 3.0, 20, 8)
+    assert_search_equal(g, o)
+    cu = np.asarray(g["cu"])
+    assert (cu > 64).sum() >= 10 and (cu <= 4096).sum() >= 10  # rows in several rounds, LDS path
+
+
 def test_tie_heavy_integers():
     """Small integer data: many exact distance ties at the K-th boundary, so the
     reference's tie rule decides which ids are returned."""
