@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
 // cost more than it saves).  The order within a bin is whatever the atomics
 // give: results do not depend on the order.
 constexpr int kLocBins = 1 << 16;
-constexpr int kLocChunk = 16;     // consecutive ordered queries per XCD claim
+constexpr int kLocChunk = 64;     // consecutive ordered queries per XCD claim
 constexpr int kLocMinQ = 8192;    // smaller batches keep batch order
 
 // Sort key of batch index i: its class, then the 12-bit coarse-cell key the
