@@ -172,37 +172,50 @@ orc_index* orc_index_build(int N, int D, int L, int M, float W, const float* A, 
     for (int m = 0; m < M; ++m) ix->b64[(size_t)t * M + m] = (double)b[(size_t)t * M + m];
   }
   ix->tables.resize(L);
-  std::vector<int32_t> codes((size_t)N * M);
-  std::vector<uint32_t> order(N);
-  for (int t = 0; t < L; ++t) {
-    const double* At = ix->At.data() + (size_t)t * D * M;
-    const double* bt = ix->b64.data() + (size_t)t * M;
+  // Tables are independent: they are processed in chunks of G (the thread
+  // count), projecting the chunk's tables over all points in parallel, then
+  // sorting and bucketing one table per thread.  Every table's result is the
+  // same as when processed alone.
+  const int G = std::max(1, std::min(L, omp_get_max_threads()));
+  std::vector<std::vector<int32_t>> codes(G);
+  for (int t0 = 0; t0 < L; t0 += G) {
+    const int nt = std::min(G, L - t0);
+    for (int g = 0; g < nt; ++g) codes[g].resize((size_t)N * M);
 #pragma omp parallel for schedule(static)
     for (long long n = 0; n < N; ++n) {
       double v[256];
-      project_row(X + (size_t)n * D, At, bt, (double)W, D, M, v);
-      for (int m = 0; m < M; ++m) codes[(size_t)n * M + m] = (int32_t)std::floor(v[m]);
-    }
-    for (int n = 0; n < N; ++n) order[n] = (uint32_t)n;
-    const int32_t* cp = codes.data();
-    std::stable_sort(order.begin(), order.end(), [cp, M](uint32_t a, uint32_t c) {
-      return lex_cmp(cp + (size_t)a * M, cp + (size_t)c * M, M) < 0;
-    });
-    Table& tb = ix->tables[t];
-    tb.total = order;
-    tb.start.clear();
-    tb.size.clear();
-    tb.codes.clear();
-    for (int i = 0; i < N; ++i) {
-      if (i == 0 || lex_cmp(cp + (size_t)order[i] * M, cp + (size_t)order[i - 1] * M, M) != 0) {
-        tb.start.push_back((uint32_t)i);
-        tb.codes.insert(tb.codes.end(), cp + (size_t)order[i] * M, cp + (size_t)order[i] * M + M);
+      for (int g = 0; g < nt; ++g) {
+        const int t = t0 + g;
+        project_row(X + (size_t)n * D, ix->At.data() + (size_t)t * D * M,
+                    ix->b64.data() + (size_t)t * M, (double)W, D, M, v);
+        for (int m = 0; m < M; ++m) codes[g][(size_t)n * M + m] = (int32_t)std::floor(v[m]);
       }
     }
-    tb.B = (int)tb.start.size();
-    tb.size.resize(tb.B);
-    for (int k = 0; k < tb.B; ++k)
-      tb.size[k] = (k + 1 < tb.B ? tb.start[k + 1] : (uint32_t)N) - tb.start[k];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < nt; ++g) {
+      const int32_t* cp = codes[g].data();
+      std::vector<uint32_t> order(N);
+      for (int n = 0; n < N; ++n) order[n] = (uint32_t)n;
+      std::stable_sort(order.begin(), order.end(), [cp, M](uint32_t a, uint32_t c) {
+        return lex_cmp(cp + (size_t)a * M, cp + (size_t)c * M, M) < 0;
+      });
+      Table& tb = ix->tables[t0 + g];
+      tb.start.clear();
+      tb.size.clear();
+      tb.codes.clear();
+      for (int i = 0; i < N; ++i) {
+        if (i == 0 || lex_cmp(cp + (size_t)order[i] * M, cp + (size_t)order[i - 1] * M, M) != 0) {
+          tb.start.push_back((uint32_t)i);
+          tb.codes.insert(tb.codes.end(), cp + (size_t)order[i] * M,
+                          cp + (size_t)order[i] * M + M);
+        }
+      }
+      tb.total = std::move(order);
+      tb.B = (int)tb.start.size();
+      tb.size.resize(tb.B);
+      for (int k = 0; k < tb.B; ++k)
+        tb.size[k] = (k + 1 < tb.B ? tb.start[k + 1] : (uint32_t)N) - tb.start[k];
+    }
   }
   return ix;
 }

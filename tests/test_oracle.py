@@ -290,3 +290,44 @@ def test_topk_ties_follow_closed_form():
 def test_topk_single_slot_takes_largest_tied_id():
     oi, od = oracle.topk_heap(np.array([1.0, 1.0, 1.0]), np.array([3, 7, 9], np.uint32), 1)
     assert oi.tolist() == [9]
+
+
+# ------------------------------------------------------- recall ground truth
+def test_bruteforce_knn_matches_numpy_argsort():
+    """orc_bruteforce_knn (the exact kNN that recall@10 is measured against)
+    equals a numpy lexsort on (exact distance, id), including exact ties
+    (small integer data) and K beyond N (padding with -1)."""
+    rng = np.random.default_rng(314)
+    for N, D, Q, K, ints in [(500, 7, 30, 10, False), (400, 3, 40, 25, True), (12, 5, 4, 20, False)]:
+        if ints:
+            X = rng.integers(0, 3, size=(N, D)).astype(np.float32)
+            Y = rng.integers(0, 3, size=(Q, D)).astype(np.float32)
+        else:
+            X = rng.standard_normal((N, D)).astype(np.float32)
+            Y = rng.standard_normal((Q, D)).astype(np.float32)
+        ids, d = oracle.bruteforce_knn(X, Y, K)
+        for q in range(Q):
+            # Exact rational distances (float inputs are dyadic rationals).
+            exact = [sum((Fraction(float(Y[q, j])) - Fraction(float(X[n, j]))) ** 2
+                         for j in range(D)) for n in range(N)]
+            order = sorted(range(N), key=lambda n: (exact[n], n))[:K]
+            k = min(K, N)
+            np.testing.assert_array_equal(ids[q, :k], order[:k])
+            ref = np.array([float(exact[n]) for n in order[:k]])
+            # The oracle sums with one FMA rounding per term: relative error
+            # at most D * 2^-53.
+            np.testing.assert_allclose(d[q, :k], ref, rtol=D * 2.0 ** -53, atol=0)
+            if K > N:
+                assert (ids[q, N:] == -1).all()
+
+
+def test_recall_counts_ties_at_the_kth_distance():
+    """recall_at_k counts a returned id when its exact distance is <= the
+    exact k-th distance, so an id tied with the k-th neighbour but not the one
+    brute force lists still counts; an id beyond it and -1 do not."""
+    X = np.array([[0.0], [1.0], [-1.0], [2.0], [5.0]], np.float32)
+    Y = np.array([[0.0]], np.float32)
+    # Exact: 0 (id 0), 1 (ids 1, 2 tied), 4 (id 3), 25 (id 4).
+    assert oracle.recall_at_k(X, Y, np.array([[0, 2]]), k=2) == 1.0
+    assert oracle.recall_at_k(X, Y, np.array([[0, 3]]), k=2) == 0.5
+    assert oracle.recall_at_k(X, Y, np.array([[-1, 1]]), k=2) == 0.5
