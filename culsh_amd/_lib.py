@@ -37,7 +37,7 @@ class SrvRequest(ctypes.Structure):
 
 class SrvConfig(ctypes.Structure):
     _fields_ = [("slots_per_index", c_int), ("batching", c_int), ("hbm_fraction", c_double),
-                ("max_batch_queries", ctypes.c_longlong)]
+                ("max_batch_queries", ctypes.c_longlong), ("no_growth", c_int)]
 
 
 class SrvStats(ctypes.Structure):
@@ -45,7 +45,9 @@ class SrvStats(ctypes.Structure):
                 ("max_ms", c_double), ("queries_per_s", c_double), ("requests_per_s", c_double),
                 ("batches", ctypes.c_longlong), ("queries", ctypes.c_longlong),
                 ("requests", ctypes.c_longlong), ("mean_batch_requests", c_double),
-                ("slot_bytes", ctypes.c_longlong), ("deferred", ctypes.c_longlong)]
+                ("slot_bytes", ctypes.c_longlong), ("deferred", ctypes.c_longlong),
+                ("split_requests", ctypes.c_longlong), ("grown_slots", ctypes.c_longlong),
+                ("host_io", c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/culsh_c.h declares.
@@ -103,12 +105,15 @@ SIGNATURES = {
     "culsh_bt_wait": (c_int, [c_vp]),
     "culsh_bt_last_ms": (c_int, [c_vp, P(c_float)]),
     "culsh_bt_rechecked": (c_ull, [c_vp]),
+    "culsh_bt_profile": (c_int, [c_vp, P(Request), c_int, c_vp, c_vp, c_vp, c_vp, P(c_float)]),
     "culsh_bt_autotune": (c_int, [c_vp, P(Request), c_int, c_vp, c_vp, c_vp, c_int, P(c_float),
                                   P(c_float)]),
     "culsh_srv_create": (c_int, [P(c_vp), c_int, P(SrvConfig), P(c_vp)]),
     "culsh_srv_destroy": (None, [c_vp]),
     "culsh_srv_run": (c_int, [c_vp, P(SrvRequest), c_int, P(c_vp), P(c_vp), P(c_vp), P(c_double),
                               P(SrvStats)]),
+    "culsh_srv_run_host": (c_int, [c_vp, P(SrvRequest), c_int, P(c_vp), P(c_vp), P(c_vp),
+                                   P(c_vp), P(c_vp), P(c_vp), P(c_double), P(SrvStats)]),
 }
 
 FORM_PLAIN = 0

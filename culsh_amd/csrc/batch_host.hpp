@@ -155,27 +155,37 @@ BatchArgs batch_args(culsh_bt* bt) {
 }
 
 // Grows the plan's buffers to hold a batch of Qtot queries, nreq requests
-// and Ptot probe slots per table; drops the graph when anything moved.
+// and Ptot probe slots per table; drops the graph when anything moves.  On
+// a failed allocation the capacities of the buffers it released are zeroed
+// (the next reserve allocates them again) and no graph is left that
+// references them.
 int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
   const int L = bt->ht->L;
-  bool moved = false;
-  if (nreq > bt->Rcap || !bt->hpack) {
+  const bool grow_r = nreq > bt->Rcap || !bt->hpack, grow_q = Qtot > bt->Qcap,
+             grow_p = Ptot > bt->Pcap, need_h = !bt->dlhist.p, need_s = !bt->dscr.p;
+  if (!(grow_r || grow_q || grow_p || need_h || need_s)) return CULSH_OK;
+  bt->drop_graph();
+  const long long Pc = std::max(Ptot, std::min(2 * bt->Pcap, ((1ll << 31) - 1) / L));
+  if (grow_p && (long long)L * Pc >= (1ll << 31))
+    return fail(CULSH_E_LIMIT, "L * (sum of Q*T over the batch) exceeds 2^31 - 1");
+  if (grow_r) {
     const int R = std::max(nreq, std::max(64, 2 * bt->Rcap));
     const size_t off_cls = (sizeof(BatchHdr) + 63) & ~size_t(63);
     const size_t off_req = (off_cls + R * sizeof(BatchClass) + 63) & ~size_t(63);
     const size_t packed = off_req + R * sizeof(BatchReq);
-    TRY(bt->dpack.exact(packed));
+    bt->Rcap = 0;
     if (bt->hpack) HIP_TRY(hipHostFree(bt->hpack));
     bt->hpack = nullptr;
+    TRY(bt->dpack.exact(packed));
     HIP_TRY(hipHostMalloc((void**)&bt->hpack, packed, hipHostMallocDefault));
     bt->Rcap = R;
     bt->off_cls = off_cls;
     bt->off_req = off_req;
     bt->packed = packed;
-    moved = true;
   }
-  if (Qtot > bt->Qcap) {
+  if (grow_q) {
     const int Qc = std::max(Qtot, std::min(2 * bt->Qcap, (1 << 30)));
+    bt->Qcap = 0;
     TRY(bt->dbq.exact(Qc));
     TRY(bt->dcnu_qt.exact((size_t)Qc * L));
     TRY(bt->dcnu_q.exact(Qc));
@@ -184,33 +194,27 @@ int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
     TRY(bt->dorder.exact(Qc));
     TRY(bt->dlkey.exact(Qc));
     bt->Qcap = Qc;
-    moved = true;
   }
-  if (Ptot > bt->Pcap) {
-    const long long Pc = std::max(Ptot, 2 * bt->Pcap);
-    if ((long long)L * Pc >= (1ll << 31))
-      return fail(CULSH_E_LIMIT, "L * (sum of Q*T over the batch) exceeds 2^31 - 1");
+  if (grow_p) {
+    bt->Pcap = 0;
     TRY(bt->dqb.exact((size_t)L * Pc));
     bt->Pcap = Pc;
-    moved = true;
   }
-  if (!bt->dlhist.p) {
-    TRY(bt->dlhist.exact(kLocBins));
-    moved = true;
-  }
-  if (!bt->dscr.p) {
+  if (need_h) TRY(bt->dlhist.exact(kLocBins));
+  if (need_s)
     TRY(bt->dscr.exact((size_t)culsh_bt::kOvGrid *
                        ((1ull << culsh_bt::kHbitsMax) + 2ull * culsh_bt::kUcapMax)));
-    moved = true;
-  }
-  if (moved) bt->drop_graph();
   return CULSH_OK;
 }
 
-// Captures: descriptor upload, expansion, persistent kernel, overflow
-// kernel, header download.  Every pointer is a plan buffer, so the graph
-// serves any batch within the capacities.
-int bt_capture(culsh_bt* bt) {
+// The plan's work on stream s: descriptor upload, expansion, the probe and
+// candidate stages, overflow kernel, header download.  Every pointer is a
+// plan buffer, so a captured copy serves any batch within the capacities.
+// With `ev` (eager runs only), ev[0..4] are recorded before the expansion,
+// after the probe stage, after the locality order, after the candidate
+// stage and at the end (the fused schedule's one kernel counts as the
+// candidate stage).
+void bt_enqueue(culsh_bt* bt, hipStream_t st, hipEvent_t* ev, hipError_t* err) {
   culsh_ht* ht = bt->ht;
   const culsh_schedule& s = bt->sched;
   const BatchArgs a = batch_args(bt);
@@ -222,43 +226,68 @@ int bt_capture(culsh_bt* bt) {
   const long long pcap = ((long long)(bt->Qcap + s.probe_queries - 1) / s.probe_queries +
                           std::min(bt->Rcap, 64)) *
                          ((ht->L + s.probe_tables - 1) / s.probe_tables);
-  if (pcap >= (1ll << 31)) return fail(CULSH_E_LIMIT, "batch probe grid too large");
-  bt->drop_graph();
-  HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
   const hipError_t m1 =
-      hipMemcpyAsync(bt->dpack.p, bt->hpack, bt->packed, hipMemcpyHostToDevice, bt->cap);
-  hipLaunchKernelGGL(k_batch_expand, dim3(egrid), dim3(256), 0, bt->cap, a.hdr,
+      hipMemcpyAsync(bt->dpack.p, bt->hpack, bt->packed, hipMemcpyHostToDevice, st);
+  mark(0);
+  hipLaunchKernelGGL(k_batch_expand, dim3(egrid), dim3(256), 0, st, a.hdr,
                      reinterpret_cast<const BatchReq*>(bt->dpack.p + bt->off_req), bt->dbq.p);
   if (s.fused) {
+    mark(1);
+    mark(2);
     hipLaunchKernelGGL(batch_kernel(ht->M, s.rows_in_flight, s.wgs_per_cu), dim3(grid),
-                       dim3(256), 0, bt->cap, a);
+                       dim3(256), 0, st, a);
   } else {
     // Wide queues first; a launch whose range is empty exits at once.
     const int plds = batch_probe_lds(ht->M);
     const dim3 pg((unsigned)pcap);
-    hipLaunchKernelGGL(batch_probe_kernel_j<8>(ht->M), pg, dim3(256), plds, bt->cap, a);
-    hipLaunchKernelGGL(batch_probe_kernel_j<4>(ht->M), pg, dim3(256), plds, bt->cap, a);
+    hipLaunchKernelGGL(batch_probe_kernel_j<8>(ht->M), pg, dim3(256), plds, st, a);
+    hipLaunchKernelGGL(batch_probe_kernel_j<4>(ht->M), pg, dim3(256), plds, st, a);
+    mark(1);
     // Locality order of the candidate stage (counting sort, fixed grids).
-    hipLaunchKernelGGL(k_batch_lzero, dim3(64), dim3(256), 0, bt->cap, a);
-    hipLaunchKernelGGL(k_batch_lhist, dim3(egrid), dim3(256), 0, bt->cap, a);
-    hipLaunchKernelGGL(k_batch_lscan, dim3(1), dim3(256), 0, bt->cap, a);
-    hipLaunchKernelGGL(k_batch_lscatter, dim3(egrid), dim3(256), 0, bt->cap, a);
+    hipLaunchKernelGGL(k_batch_lzero, dim3(64), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_batch_lhist, dim3(egrid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_batch_lscan, dim3(1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_batch_lscatter, dim3(egrid), dim3(256), 0, st, a);
+    mark(2);
     if (s.wgs_per_cu == 0) {
       // One workgroup per query position, chunks interleaved over the XCDs.
       const long long per = (bt->Qcap + 8ll * kLocChunk - 1) / (8ll * kLocChunk) * kLocChunk;
       hipLaunchKernelGGL(batch_cand_each_kernel(s.rows_in_flight), dim3((unsigned)(8 * per)),
-                         dim3(256), 0, bt->cap, a);
+                         dim3(256), 0, st, a);
     } else {
-      hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0,
-                         bt->cap, a);
+      hipLaunchKernelGGL(batch_cand_kernel(s.rows_in_flight), dim3(grid), dim3(256), 0, st, a);
     }
   }
-  hipLaunchKernelGGL(k_batch_overflow, dim3(culsh_bt::kOvGrid), dim3(256), 0, bt->cap, a,
-                     bt->dscr.p, culsh_bt::kHbitsMax, culsh_bt::kUcapMax);
+  mark(3);
+  hipLaunchKernelGGL(k_batch_overflow, dim3(culsh_bt::kOvGrid), dim3(256), 0, st, a, bt->dscr.p,
+                     culsh_bt::kHbitsMax, culsh_bt::kUcapMax);
   const hipError_t m2 =
-      hipMemcpyAsync(bt->hres, bt->dpack.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, bt->cap);
+      hipMemcpyAsync(bt->hres, bt->dpack.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, st);
+  mark(4);
   hipError_t le = hipGetLastError();
   if (le == hipSuccess) le = m1 != hipSuccess ? m1 : m2;
+  *err = le;
+}
+
+int bt_check_grid(culsh_bt* bt) {
+  const culsh_schedule& s = bt->sched;
+  const long long pcap = ((long long)(bt->Qcap + s.probe_queries - 1) / s.probe_queries +
+                          std::min(bt->Rcap, 64)) *
+                         ((bt->ht->L + s.probe_tables - 1) / s.probe_tables);
+  if (pcap >= (1ll << 31)) return fail(CULSH_E_LIMIT, "batch probe grid too large");
+  return CULSH_OK;
+}
+
+// Captures the plan's work into a graph on the capture stream.
+int bt_capture(culsh_bt* bt) {
+  TRY(bt_check_grid(bt));
+  bt->drop_graph();
+  HIP_TRY(hipStreamBeginCapture(bt->cap, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  bt_enqueue(bt, bt->cap, nullptr, &le);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(bt->cap, &g);
   if (le != hipSuccess || ce != hipSuccess) {
@@ -268,7 +297,7 @@ int bt_capture(culsh_bt* bt) {
   }
   bt->graph = g;
   HIP_TRY(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
-  bt->graph_sched = s;
+  bt->graph_sched = bt->sched;
   return CULSH_OK;
 }
 
@@ -286,8 +315,12 @@ int bt_wait(culsh_bt* bt) {
   return CULSH_OK;
 }
 
-int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ, int* d_ids,
-              float* d_dists, hipStream_t stream) {
+// Lays the batch out (batch order, classes, work items) into the pinned
+// descriptor and makes sure the plan's buffers (and, with `capture`, its
+// graph) fit it.  *empty is set when no request has queries.
+int bt_prepare(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ, int* d_ids,
+               float* d_dists, bool capture, bool* empty) {
+  *empty = false;
   culsh_ht* ht = bt->ht;
   if (!ht->indexed || !ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
   if (nreq <= 0 || !reqs) return fail(CULSH_E_ARG, "empty batch");
@@ -307,7 +340,10 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
     Qsum += q.Q;
   }
   if (Qsum >= (1ll << 30)) return fail(CULSH_E_LIMIT, "a batch holds 2^30 or more queries");
-  if (ord.empty()) return CULSH_OK;
+  if (ord.empty()) {
+    *empty = true;
+    return CULSH_OK;
+  }
   std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
     if (reqs[x].T != reqs[y].T) return reqs[x].T > reqs[y].T;
     return reqs[x].K > reqs[y].K;
@@ -350,7 +386,7 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
   item += (vc.back().n + bq - 1) / bq;
   pitem += (vc.back().n + s.probe_queries - 1) / s.probe_queries * ntg;
   TRY(bt_reserve(bt, b, n, p));
-  if (!bt->exec || !same_schedule(bt->graph_sched, s)) TRY(bt_capture(bt));
+  if (capture && (!bt->exec || !same_schedule(bt->graph_sched, s))) TRY(bt_capture(bt));
   BatchHdr h;
   std::memset(&h, 0, sizeof(h));
   h.Qm = dQ;
@@ -381,6 +417,14 @@ int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ
   std::memcpy(bt->hpack, &h, sizeof(h));
   std::memcpy(bt->hpack + bt->off_cls, vc.data(), vc.size() * sizeof(BatchClass));
   std::memcpy(bt->hpack + bt->off_req, vr.data(), vr.size() * sizeof(BatchReq));
+  return CULSH_OK;
+}
+
+int bt_launch(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* dQ, int* d_ids,
+              float* d_dists, hipStream_t stream) {
+  bool empty = false;
+  TRY(bt_prepare(bt, reqs, nreq, dQ, d_ids, d_dists, true, &empty));
+  if (empty) return CULSH_OK;
   HIP_TRY(hipEventRecord(bt->ev0, stream));
   HIP_TRY(hipGraphLaunch(bt->exec, stream));
   HIP_TRY(hipEventRecord(bt->ev1, stream));
@@ -448,6 +492,37 @@ int culsh_bt_last_ms(culsh_bt* bt, float* ms) {
 
 unsigned long long culsh_bt_rechecked(culsh_bt* bt) { return bt ? bt->rechecked : 0ull; }
 
+int culsh_bt_profile(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
+                     int* d_ids, float* d_dists, void* stream, float* stage_ms) {
+  if (!bt || !stage_ms) return fail(CULSH_E_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  bool empty = false;
+  TRY(bt_prepare(bt, reqs, nreq, d_queries, d_ids, d_dists, false, &empty));
+  for (int i = 0; i < 4; ++i) stage_ms[i] = 0.f;
+  if (empty) return CULSH_OK;
+  TRY(bt_check_grid(bt));
+  hipEvent_t ev[5] = {};
+  auto done = [&](int rc) {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    return rc;
+  };
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return done(fail(CULSH_E_DEVICE, "hipEventCreate"));
+  hipError_t le = hipSuccess;
+  bt_enqueue(bt, st, ev, &le);
+  if (le != hipSuccess) return done(fail(CULSH_E_DEVICE, "batch launch: %s", hipGetErrorString(le)));
+  if (hipEventSynchronize(ev[4]) != hipSuccess) return done(fail(CULSH_E_DEVICE, "batch sync"));
+  for (int i = 0; i < 4; ++i)
+    if (hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]) != hipSuccess)
+      return done(fail(CULSH_E_DEVICE, "hipEventElapsedTime"));
+  const BatchHdr& r = *bt->hres;
+  bt->rechecked = r.n_recheck;
+  if (r.err & 1)
+    return done(fail(CULSH_E_LIMIT, "batch: a query exceeded the candidate capacities"));
+  return done(CULSH_OK);
+}
+
 int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
                       int* d_ids, float* d_dists, int reps, float* default_ms, float* tuned_ms) {
   if (!bt) return fail(CULSH_E_ARG, "null batch");
@@ -463,7 +538,10 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
     }
     return CULSH_OK;
   };
-  const culsh_schedule def{32, 1, 32, 4, 3, 0};
+  // The starting point is a fixed naive schedule (one fused kernel, 32x1
+  // probe tile, 32 queries per item, 4 rows in flight, one workgroup per
+  // CU), not the library's default, so tuned vs default measures the search.
+  const culsh_schedule def{32, 1, 32, 4, 1, 1};
   culsh_schedule best = def;
   float tbest = 0.f;
   TRY(timed(def, &tbest));
@@ -472,6 +550,7 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
   const int qpis[4] = {8, 16, 32, 64};
   const int rows[3] = {2, 4, 8};
   const int wgs[4] = {0, 1, 2, 3};
+  bool improved = false;
   auto consider = [&](const culsh_schedule& s) -> int {
     if (same_schedule(s, best)) return CULSH_OK;
     float t = 0.f;
@@ -479,38 +558,44 @@ int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const f
     if (t < 0.99f * tbest) {
       tbest = t;
       best = s;
+      improved = true;
     }
     return CULSH_OK;
   };
-  for (auto& tl : tiles) {
-    culsh_schedule s = best;
-    s.probe_queries = tl[0];
-    s.probe_tables = tl[1];
-    s.queries_per_item = std::max(tl[0], s.queries_per_item / tl[0] * tl[0]);
-    TRY(consider(s));
-  }
-  for (int v : qpis) {
-    culsh_schedule s = best;
-    if (v < s.probe_queries) continue;
-    s.queries_per_item = v;
-    TRY(consider(s));
-  }
-  for (int v : rows) {
-    culsh_schedule s = best;
-    s.rows_in_flight = v;
-    TRY(consider(s));
-  }
-  for (int v : wgs) {
-    culsh_schedule s = best;
-    s.wgs_per_cu = v;
-    if (s.fused && v == 0) continue;
-    TRY(consider(s));
-  }
-  for (int f = 0; f < 2; ++f) {
-    culsh_schedule s = best;
-    s.fused = f;
-    if (f && s.wgs_per_cu == 0) s.wgs_per_cu = 3;
-    TRY(consider(s));
+  // Coordinate sweeps until one changes nothing (at most three).
+  for (int pass = 0; pass < 3; ++pass) {
+    improved = false;
+    for (int f = 0; f < 2; ++f) {
+      culsh_schedule s = best;
+      s.fused = f;
+      if (f && s.wgs_per_cu == 0) s.wgs_per_cu = 3;
+      TRY(consider(s));
+    }
+    for (int v : wgs) {
+      culsh_schedule s = best;
+      s.wgs_per_cu = v;
+      if (s.fused && v == 0) continue;
+      TRY(consider(s));
+    }
+    for (auto& tl : tiles) {
+      culsh_schedule s = best;
+      s.probe_queries = tl[0];
+      s.probe_tables = tl[1];
+      s.queries_per_item = std::max(tl[0], s.queries_per_item / tl[0] * tl[0]);
+      TRY(consider(s));
+    }
+    for (int v : qpis) {
+      culsh_schedule s = best;
+      if (v < s.probe_queries) continue;
+      s.queries_per_item = v;
+      TRY(consider(s));
+    }
+    for (int v : rows) {
+      culsh_schedule s = best;
+      s.rows_in_flight = v;
+      TRY(consider(s));
+    }
+    if (!improved) break;
   }
   TRY(culsh_bt_set_schedule(bt, &best));
   if (tuned_ms) *tuned_ms = tbest;

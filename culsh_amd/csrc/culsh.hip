@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -98,6 +99,13 @@ int floor_log2_host(long long x) {
   return l;
 }
 
+// Generation stamps of dataset contents on the device: every index build or
+// load and every upload of a search dataset takes a new one, so a cache of
+// per-row data keyed on (pointer, generation) never outlives its dataset
+// even when an allocation is reused at the same address.
+std::atomic<unsigned long long> g_data_gen{0};
+unsigned long long next_data_gen() { return ++g_data_gen; }
+
 inline unsigned grid_for(long long n, int block = 256) {
   return (unsigned)((n + block - 1) / block);
 }
@@ -136,6 +144,7 @@ struct culsh_ht {
   DevBuf<float> dX_own;
   const float* dX = nullptr;          // resident dataset [N][D]
   const float* hX_indexed = nullptr;  // host pointer index() was given
+  unsigned long long data_gen = 0;    // generation of the resident dataset
   DevBuf<BucketRec> recs;             // [B_total] sorted by key: key, start, size
   DevBuf<uint32_t> sids;              // [L*N] members, bucket by bucket
   DevBuf<int16_t> bcodes;             // [B_total][M]
@@ -159,6 +168,7 @@ struct culsh_ht {
   void drop_index() {
     indexed = false;
     canon = false;
+    data_gen = next_data_gen();
     dX_own.release();
     dX = nullptr;
     hX_indexed = nullptr;
@@ -586,6 +596,7 @@ static int index_common(culsh_ht* ht, const float* matrix, bool device, FILE* db
     ht->dX = ht->dX_own.p;
     ht->hX_indexed = matrix;
   }
+  ht->data_gen = next_data_gen();
   int rc = index_resident(ht, dbg);
   if (rc != CULSH_OK) {
     std::string keep = g_err;
@@ -886,10 +897,20 @@ struct culsh_st {
   DevBuf<uint64_t> dlk0, dlk1;
   DevBuf<uint32_t> dlv0, dlv1, dlscr, dgscratch;
   bool gscratch_clean = false;  // group hash slots hold their empty state
-  // |x|^2 and max |x| of the dataset the last precision search used.
+  size_t g_region = 0;           // ... for this per-workgroup region size
+  int g_hbits = 0;               // ... and hash size
+  // A search dataset uploaded by search() (when it is not the index's
+  // resident one): the host pointer, index and index generation it was
+  // uploaded for, and its own data generation.
+  const float* alt_host = nullptr;
+  const culsh_ht* alt_ht = nullptr;
+  unsigned long long alt_ht_gen = 0, alt_gen = 0;
+  // |x|^2 and max |x| of the dataset the last precision search used, keyed
+  // on (device pointer, data generation).
   DevBuf<float> dxn2;
   DevBuf<unsigned> dxmax;
   const float* xn2_for = nullptr;
+  unsigned long long xn2_gen = 0;
   float xmax = 0.f;
   std::vector<int> ids;
   std::vector<float> dists;
@@ -1025,7 +1046,8 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
   culsh_ht* ht = st->ht;
   const int L = ht->L, D = ht->D, T = st->T, K = st->K;
   const long long N = ht->N;
-  if (st->xn2_for != dX) {
+  const unsigned long long gen = dX == ht->dX ? ht->data_gen : st->alt_gen;
+  if (st->xn2_for != dX || st->xn2_gen != gen) {
     TRY(st->dxn2.ensure(N));
     TRY(st->dxmax.ensure(1));
     HIP_TRY(hipMemsetAsync(st->dxmax.p, 0, 4, s));
@@ -1036,6 +1058,7 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
     HIP_TRY(hipStreamSynchronize(s));
     std::memcpy(&st->xmax, &bits, 4);
     st->xn2_for = dX;
+    st->xn2_gen = gen;
   }
   const uint32_t* order = nullptr;
   TRY(locality_order(st, dQ, Q, s, &order));
@@ -1048,7 +1071,9 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
   HIP_TRY(hipStreamSynchronize(s));
   if (gmax >= (1ull << 29))
     return fail(CULSH_E_LIMIT, "a query group has %llu candidates (limit 2^29)", gmax);
-  const unsigned qcap = (unsigned)std::max<unsigned long long>(gmax, 1);
+  // Capacity rounded to a power of two, so that the scratch layout (which
+  // the clean state below belongs to) changes rarely between searches.
+  const unsigned qcap = 1u << ceil_log2_host((long long)std::max<unsigned long long>(gmax, 1));
   const int hbits = std::max(8, ceil_log2_host(2ll * qcap));
   const size_t region = 2 * (1ull << hbits) + 5ull * qcap;
   const size_t budget = (4ull << 30) / 4;
@@ -1058,10 +1083,16 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
     TRY(st->dgscratch.exact(region * G));
     st->gscratch_clean = false;
   }
+  // Each group leaves the hash ids and masks of its workgroup's region
+  // all-ones, but only for the layout (region, hbits) it ran with: another
+  // layout puts hash slots over words of the other arrays.
+  if (st->g_region != region || st->g_hbits != hbits) st->gscratch_clean = false;
   if (!st->gscratch_clean) {
     // Empty hash ids and complemented masks are all-ones words.
     HIP_TRY(hipMemsetAsync(st->dgscratch.p, 0xFF, st->dgscratch.n * 4, s));
     st->gscratch_clean = true;
+    st->g_region = region;
+    st->g_hbits = hbits;
   }
   hipLaunchKernelGGL(k_group_candidates, dim3(G), dim3(256), 0, s, dX,
                      (const float*)st->dxn2.p, st->xmax, dQ, Q, D, L, T, K,
@@ -1079,6 +1110,9 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
 int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
                hipStream_t s, bool timed, bool wait = true) {
   TRY(st_check_async(st, wait));
+  // A device search still running on another stream uses the same working
+  // buffers: order this one after it.
+  if (st->pending) HIP_TRY(hipStreamWaitEvent(s, st->evdone, 0));
   // The plain form can finish without the host: its overflowed queries go
   // to a fixed-size device kernel.  The precision form sizes its group
   // working sets on the host.
@@ -1250,9 +1284,20 @@ int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* data
   TRY(ensure_canonical(ht));
   const float* dX = ht->dX;
   if (!dX || dataset != ht->hX_indexed) {
-    // A dataset other than the resident one: upload it for the distances.
-    TRY(st->dX_alt.ensure((size_t)ht->N * D));
-    HIP_TRY(hipMemcpy(st->dX_alt.p, dataset, (size_t)ht->N * D * 4, hipMemcpyHostToDevice));
+    // A dataset other than the resident one (an index built from a device
+    // pointer or loaded from a file, or another host array): uploaded for
+    // the distances on the first search that passes this host pointer for
+    // this index, then reused while the pointer and the index stay the same.
+    if (!st->dX_alt.p || st->alt_host != dataset || st->alt_ht != ht ||
+        st->alt_ht_gen != ht->data_gen) {
+      st->alt_host = nullptr;
+      TRY(st->dX_alt.ensure((size_t)ht->N * D));
+      HIP_TRY(hipMemcpy(st->dX_alt.p, dataset, (size_t)ht->N * D * 4, hipMemcpyHostToDevice));
+      st->alt_host = dataset;
+      st->alt_ht = ht;
+      st->alt_ht_gen = ht->data_gen;
+      st->alt_gen = next_data_gen();
+    }
     dX = st->dX_alt.p;
   }
   TRY(st->dQ.ensure((size_t)Q * D));

@@ -4,30 +4,41 @@
 // launch path it drives).
 //
 // Each index owns `slots_per_index` slots; a slot is a culsh_bt with its own
-// stream, buffers sized once at creation from the free HBM, and a captured
-// graph.  The loop moves arrived requests into per-index FIFO queues; a free
+// stream, plan buffers and a captured graph, sized at creation from the free
+// HBM.  The loop moves arrived requests into per-index FIFO queues; a free
 // slot takes every queued request of its index that still fits its
 // capacity (queries, probe slots, requests) -- head-of-line order, so a
-// request that does not fit waits for the next free slot -- and launches the
-// batch at once.  Batches of different slots overlap on the device.
-// Completion times come from the slots' HIP events, measured against an
-// event recorded when the run starts, so latencies are on the GPU timeline.
+// request that does not fit behind others waits for the next free slot --
+// and launches the batch at once.  A request larger than an empty slot
+// grows that slot when the HBM free at that moment allows it, or else runs
+// in pieces over successive batches.  Batches of different slots overlap on
+// the device.  Completion times come from the slots' HIP events, measured
+// against an event recorded when the run starts, so latencies are on the
+// GPU timeline; with host IO they include the query upload and the result
+// download of each batch.
 #pragma once
 #include <chrono>
 #include <deque>
 
 struct culsh_srv {
+  struct Piece {
+    int r;       // request
+    int q0, nq;  // its queries [q0, q0 + nq)
+  };
   struct Slot {
     culsh_bt* bt = nullptr;
     hipStream_t stream = nullptr;
+    hipEvent_t io_done = nullptr;  // after the result download (host IO runs)
     bool busy = false;
-    std::vector<int> reqs;  // request ids of the batch in flight
+    int Qcap = 0, Rcap = 0;  // this slot's capacities (it may grow)
+    long long Pcap = 0;
+    std::vector<Piece> pieces;  // the batch in flight
   };
   std::vector<culsh_ht*> ht;
   culsh_srv_config cfg{};
   std::vector<std::vector<Slot>> slots;  // [index][slot]
-  int Qcap = 0, Rcap = 0;
-  long long Pcap = 0, slot_bytes = 0;
+  int maxL = 0;
+  long long slot_bytes = 0;  // HBM of one slot at its initial capacity
   // Plain mode: one SearchTables per index.
   std::vector<culsh_st*> st;
   hipStream_t plain_stream = nullptr;
@@ -37,6 +48,7 @@ struct culsh_srv {
     for (auto& v : slots)
       for (auto& s : v) {
         if (s.bt) culsh_bt_destroy(s.bt);
+        if (s.io_done) (void)hipEventDestroy(s.io_done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
       }
     for (auto* p : st) culsh_st_destroy(p);
@@ -66,9 +78,13 @@ double pct(std::vector<double> v, double p) {
   return v[lo] + (pos - (double)lo) * (v[hi] - v[lo]);
 }
 
+struct SrvCounters {
+  long long batches = 0, batched_pieces = 0, deferred = 0, split = 0, grown = 0;
+};
+
 void fill_stats(const std::vector<double>& lat, const culsh_srv_request* reqs, int n,
-                double last_done_ms, long long batches, long long batched_reqs, long long deferred,
-                long long slot_bytes, culsh_srv_stats* s) {
+                double last_done_ms, const SrvCounters& c, long long slot_bytes, bool host_io,
+                culsh_srv_stats* s) {
   std::memset(s, 0, sizeof(*s));
   s->p50_ms = pct(lat, 0.50);
   s->p99_ms = pct(lat, 0.99);
@@ -84,12 +100,15 @@ void fill_stats(const std::vector<double>& lat, const culsh_srv_request* reqs, i
   const double span_s = (last_done_ms - (n ? reqs[0].arrival_s * 1e3 : 0.0)) * 1e-3;
   s->queries_per_s = span_s > 0 ? (double)q / span_s : 0.0;
   s->requests_per_s = span_s > 0 ? (double)n / span_s : 0.0;
-  s->batches = batches;
+  s->batches = c.batches;
   s->queries = q;
   s->requests = n;
-  s->mean_batch_requests = batches ? (double)batched_reqs / (double)batches : 0.0;
+  s->mean_batch_requests = c.batches ? (double)c.batched_pieces / (double)c.batches : 0.0;
   s->slot_bytes = slot_bytes;
-  s->deferred = deferred;
+  s->deferred = c.deferred;
+  s->split_requests = c.split;
+  s->grown_slots = c.grown;
+  s->host_io = host_io ? 1 : 0;
 }
 
 int srv_check(culsh_srv* srv, const culsh_srv_request* reqs, int n) {
@@ -102,18 +121,43 @@ int srv_check(culsh_srv* srv, const culsh_srv_request* reqs, int n) {
       return fail(CULSH_E_ARG, "request %d: Q, K > 0, 0 < T <= 64, offsets >= 0 required", r);
     if (q.arrival_s < prev) return fail(CULSH_E_ARG, "requests must be sorted by arrival");
     prev = q.arrival_s;
-    if (srv->cfg.batching &&
-        (q.Q > srv->Qcap || (long long)q.Q * q.T > srv->Pcap))
-      return fail(CULSH_E_LIMIT, "request %d (Q=%d, T=%d) exceeds a slot's capacity", r, q.Q,
-                  q.T);
   }
+  return CULSH_OK;
+}
+
+// Host arrays of a run with query upload and result download (all null for
+// a run on device-resident queries and results).
+struct HostIO {
+  const float* const* q = nullptr;
+  int* const* ids = nullptr;
+  float* const* dists = nullptr;
+  bool on() const { return q != nullptr; }
+};
+
+// Query rows [q0, q0 + nq) of request r: host -> device.
+int io_upload(const HostIO& io, const culsh_srv_request& q, int D, int q0, int nq,
+              const float* const* dQ, hipStream_t s) {
+  const size_t off = (size_t)(q.q_row + q0) * D;
+  HIP_TRY(hipMemcpyAsync(const_cast<float*>(dQ[q.index]) + off, io.q[q.index] + off,
+                         (size_t)nq * D * 4, hipMemcpyHostToDevice, s));
+  return CULSH_OK;
+}
+
+// Results of queries [q0, q0 + nq) of request r: device -> host.
+int io_download(const HostIO& io, const culsh_srv_request& q, int q0, int nq,
+                int* const* d_ids, float* const* d_dists, hipStream_t s) {
+  const size_t off = (size_t)q.out_offset + (size_t)q0 * q.K, cnt = (size_t)nq * q.K;
+  HIP_TRY(hipMemcpyAsync(io.ids[q.index] + off, d_ids[q.index] + off, cnt * 4,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(io.dists[q.index] + off, d_dists[q.index] + off, cnt * 4,
+                         hipMemcpyDeviceToHost, s));
   return CULSH_OK;
 }
 
 // One request at a time through the search path (the plain baseline).
 int srv_run_plain(culsh_srv* srv, const culsh_srv_request* reqs, int n,
                   const float* const* dQ, int* const* d_ids, float* const* d_dists,
-                  std::vector<double>& lat, double* last_done_ms) {
+                  const HostIO& io, std::vector<double>& lat, double* last_done_ms) {
   using clk = std::chrono::steady_clock;
   hipStream_t s = srv->plain_stream;
   HIP_TRY(hipEventRecord(srv->start, s));
@@ -126,8 +170,10 @@ int srv_run_plain(culsh_srv* srv, const culsh_srv_request* reqs, int n,
     culsh_st* st = srv->st[q.index];
     culsh_ht* ht = srv->ht[q.index];
     TRY(culsh_st_reset(st, ht, q.K, q.T));
+    if (io.on()) TRY(io_upload(io, q, ht->D, 0, q.Q, dQ, s));
     TRY(run_search(st, dQ[q.index] + q.q_row * ht->D, ht->dX, q.Q, d_ids[q.index] + q.out_offset,
                    d_dists[q.index] + q.out_offset, s, false));
+    if (io.on()) TRY(io_download(io, q, 0, q.Q, d_ids, d_dists, s));
     HIP_TRY(hipEventRecord(srv->plain_done, s));
     HIP_TRY(hipEventSynchronize(srv->plain_done));
     float ms = 0.f;
@@ -138,13 +184,59 @@ int srv_run_plain(culsh_srv* srv, const culsh_srv_request* reqs, int n,
   return CULSH_OK;
 }
 
-int srv_run_batched(culsh_srv* srv, const culsh_srv_request* reqs, int n,
-                    const float* const* dQ, int* const* d_ids, float* const* d_dists,
-                    std::vector<double>& lat, double* last_done_ms, long long* batches,
-                    long long* batched_reqs, long long* deferred) {
+// Waits for every slot's batch in flight and frees the slots, keeping the
+// current error message: a run starts, and a failed run ends, with no batch
+// of it left on the device.
+void srv_drain(culsh_srv* srv) {
+  const std::string keep = g_err;
+  for (auto& v : srv->slots)
+    for (auto& sl : v) {
+      if (sl.bt && sl.bt->pending) (void)bt_wait(sl.bt);
+      if (sl.busy && sl.stream) (void)hipStreamSynchronize(sl.stream);
+      sl.busy = false;
+      sl.pieces.clear();
+    }
+  g_err = keep;
+}
+
+// Grows a free slot so that one batch can hold Qn queries and Pn probe
+// slots per table, when the HBM free at this moment (times the configured
+// fraction) covers the extra buffers.  Returns false (nothing changed) when
+// it does not, or the probe slots would exceed the 31-bit addressing.
+bool srv_try_grow(culsh_srv* srv, culsh_srv::Slot& sl, long long Qn, long long Pn, int* rc) {
+  *rc = CULSH_OK;
+  if (srv->cfg.no_growth) return false;
+  const int L = sl.bt->ht->L;
+  Qn = std::max<long long>(Qn, 2ll * sl.Qcap);
+  Pn = std::max<long long>(Pn, 2ll * sl.Pcap);
+  if (Qn >= (1ll << 30) || (long long)L * Pn >= (1ll << 31)) {
+    Qn = std::min<long long>(Qn, (1ll << 30) - 1);
+    Pn = std::min<long long>(Pn, ((1ll << 31) - 1) / L);
+  }
+  if (Qn <= sl.Qcap && Pn <= sl.Pcap) return false;
+  const int Rn = std::max(sl.Rcap, (int)std::min<long long>(Qn, 4096));
+  const long long extra = slot_bytes_for(L, Qn, Pn, Rn) - slot_bytes_for(L, sl.Qcap, sl.Pcap, sl.Rcap);
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return false;
+  if ((double)extra > (double)freeb * srv->cfg.hbm_fraction) return false;
+  *rc = bt_reserve(sl.bt, (int)Qn, Rn, Pn);
+  if (*rc == CULSH_OK) *rc = bt_capture(sl.bt);
+  if (*rc != CULSH_OK) return false;
+  sl.Qcap = (int)Qn;
+  sl.Pcap = Pn;
+  sl.Rcap = Rn;
+  return true;
+}
+
+int srv_run_batched_loop(culsh_srv* srv, const culsh_srv_request* reqs, int n,
+                         const float* const* dQ, int* const* d_ids, float* const* d_dists,
+                         const HostIO& io, std::vector<double>& lat, double* last_done_ms,
+                         SrvCounters& c) {
   using clk = std::chrono::steady_clock;
   const int nidx = (int)srv->ht.size();
-  std::vector<std::deque<int>> queue(nidx);
+  std::vector<std::deque<culsh_srv::Piece>> queue(nidx);
+  std::vector<int> left(n);  // queries of a request not yet completed
+  for (int r = 0; r < n; ++r) left[r] = reqs[r].Q;
   std::vector<culsh_request> breq;
   HIP_TRY(hipEventRecord(srv->start, srv->slots[0][0].stream));
   HIP_TRY(hipEventSynchronize(srv->start));
@@ -153,59 +245,130 @@ int srv_run_batched(culsh_srv* srv, const culsh_srv_request* reqs, int n,
   while (done < n) {
     const double now = std::chrono::duration<double>(clk::now() - t0).count();
     while (next < n && reqs[next].arrival_s <= now) {
-      queue[reqs[next].index].push_back(next);
+      queue[reqs[next].index].push_back({next, 0, reqs[next].Q});
       ++next;
     }
-    // Completions.
+    // Completions: a request is done when its last piece is.
     for (int i = 0; i < nidx; ++i)
       for (auto& sl : srv->slots[i]) {
         if (!sl.busy) continue;
-        const hipError_t q = hipEventQuery(sl.bt->ev1);
+        hipEvent_t ev = io.on() ? sl.io_done : sl.bt->ev1;
+        const hipError_t q = hipEventQuery(ev);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) HIP_TRY(q);
         TRY(bt_wait(sl.bt));
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, srv->start, sl.bt->ev1));
-        for (int r : sl.reqs) lat[r] = (double)ms - reqs[r].arrival_s * 1e3;
+        HIP_TRY(hipEventElapsedTime(&ms, srv->start, ev));
+        for (const auto& pc : sl.pieces) {
+          left[pc.r] -= pc.nq;
+          if (left[pc.r] == 0) {
+            lat[pc.r] = (double)ms - reqs[pc.r].arrival_s * 1e3;
+            ++done;
+          }
+        }
         *last_done_ms = std::max(*last_done_ms, (double)ms);
-        done += (int)sl.reqs.size();
-        sl.reqs.clear();
+        sl.pieces.clear();
         sl.busy = false;
       }
-    // Dispatch: every free slot takes what fits of its index's queue.
+    // Dispatch: every free slot takes what fits of its index's queue, in
+    // arrival order.  A request that does not fit behind others waits for
+    // the next free slot; one that does not fit an empty slot grows the
+    // slot (when the free HBM allows) or runs in pieces over several
+    // batches (results per query are independent of the batch).
     for (int i = 0; i < nidx; ++i)
       for (auto& sl : srv->slots[i]) {
         if (sl.busy || queue[i].empty()) continue;
         long long qsum = 0, psum = 0;
         breq.clear();
         while (!queue[i].empty()) {
-          const culsh_srv_request& q = reqs[queue[i].front()];
-          if (qsum + q.Q > srv->Qcap || psum + (long long)q.Q * q.T > srv->Pcap ||
-              (int)breq.size() + 1 > srv->Rcap) {
-            ++*deferred;
+          culsh_srv::Piece& pc = queue[i].front();
+          const culsh_srv_request& q = reqs[pc.r];
+          long long fit = std::min<long long>(sl.Qcap - qsum, (sl.Pcap - psum) / q.T);
+          if ((int)breq.size() + 1 > sl.Rcap) fit = 0;
+          int take = 0;
+          if (fit >= pc.nq) {
+            take = pc.nq;
+          } else if (breq.empty()) {
+            int rc = CULSH_OK;
+            if (srv_try_grow(srv, sl, pc.nq, (long long)pc.nq * q.T, &rc)) {
+              ++c.grown;
+              continue;  // re-evaluate against the new capacity
+            }
+            TRY(rc);
+            take = (int)std::max<long long>(1, fit);
+            ++c.split;
+          } else {
+            ++c.deferred;
             break;
           }
           culsh_request b;
-          b.Q = q.Q;
+          b.Q = take;
           b.K = q.K;
           b.T = q.T;
           b.reserved = 0;
-          b.q_row = q.q_row;
-          b.out_offset = q.out_offset;
+          b.q_row = q.q_row + pc.q0;
+          b.out_offset = q.out_offset + (long long)pc.q0 * q.K;
           breq.push_back(b);
-          sl.reqs.push_back(queue[i].front());
-          queue[i].pop_front();
-          qsum += q.Q;
-          psum += (long long)q.Q * q.T;
+          sl.pieces.push_back({pc.r, pc.q0, take});
+          qsum += take;
+          psum += (long long)take * q.T;
+          if (take == pc.nq) {
+            queue[i].pop_front();
+          } else {
+            pc.q0 += take;
+            pc.nq -= take;
+            break;
+          }
         }
+        if (breq.empty()) continue;
+        sl.busy = true;  // from here on the slot holds work of this run
+        if (io.on())
+          for (const auto& pc : sl.pieces)
+            TRY(io_upload(io, reqs[pc.r], srv->ht[i]->D, pc.q0, pc.nq, dQ, sl.stream));
         TRY(bt_launch(sl.bt, breq.data(), (int)breq.size(), dQ[i], d_ids[i], d_dists[i],
                       sl.stream));
-        sl.busy = true;
-        ++*batches;
-        *batched_reqs += (long long)breq.size();
+        if (io.on()) {
+          for (const auto& pc : sl.pieces)
+            TRY(io_download(io, reqs[pc.r], pc.q0, pc.nq, d_ids, d_dists, sl.stream));
+          HIP_TRY(hipEventRecord(sl.io_done, sl.stream));
+        }
+        ++c.batches;
+        c.batched_pieces += (long long)breq.size();
       }
     std::this_thread::yield();
   }
+  return CULSH_OK;
+}
+
+int srv_run_batched(culsh_srv* srv, const culsh_srv_request* reqs, int n, const float* const* dQ,
+                    int* const* d_ids, float* const* d_dists, const HostIO& io,
+                    std::vector<double>& lat, double* last_done_ms, SrvCounters& c) {
+  srv_drain(srv);
+  const int rc =
+      srv_run_batched_loop(srv, reqs, n, dQ, d_ids, d_dists, io, lat, last_done_ms, c);
+  if (rc != CULSH_OK) srv_drain(srv);
+  return rc;
+}
+
+int srv_run(culsh_srv* srv, const culsh_srv_request* reqs, int n, const float* const* d_queries,
+            int* const* d_ids, float* const* d_dists, const HostIO& io, double* latency_ms,
+            culsh_srv_stats* stats) {
+  if (!srv || (!reqs && n > 0) || !d_queries || !d_ids || !d_dists)
+    return fail(CULSH_E_ARG, "invalid arguments");
+  if (n == 0) return CULSH_OK;
+  TRY(srv_check(srv, reqs, n));
+  std::vector<double> lat(n, 0.0);
+  double last = 0.0;
+  SrvCounters c;
+  if (srv->cfg.batching) {
+    TRY(srv_run_batched(srv, reqs, n, d_queries, d_ids, d_dists, io, lat, &last, c));
+  } else {
+    TRY(srv_run_plain(srv, reqs, n, d_queries, d_ids, d_dists, io, lat, &last));
+    c.batches = n;
+    c.batched_pieces = n;
+  }
+  if (latency_ms) std::copy(lat.begin(), lat.end(), latency_ms);
+  if (stats) fill_stats(lat, reqs, n, last, c, srv->slot_bytes, io.on(), stats);
   return CULSH_OK;
 }
 
@@ -239,7 +402,8 @@ int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_confi
   int maxL = 0;
   for (auto* h : srv->ht) maxL = std::max(maxL, h->L);
   if (cfg->batching) {
-    // Slot capacity from free HBM: Q queries and 32 Q probe slots per table.
+    // Initial slot capacity from the free HBM now: Q queries and 32 Q probe
+    // slots per table.  Slots grow later by the free HBM at that moment.
     size_t freeb = 0, totalb = 0;
     e = hipMemGetInfo(&freeb, &totalb);
     if (e != hipSuccess) return bail(fail(CULSH_E_DEVICE, "hipMemGetInfo: %s", hipGetErrorString(e)));
@@ -254,9 +418,7 @@ int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_confi
     if (need > share)
       return bail(fail(CULSH_E_MEMORY, "free HBM share %lld B is below one slot's minimum",
                        share));
-    srv->Qcap = (int)Q;
-    srv->Pcap = 32 * Q;
-    srv->Rcap = (int)std::min<long long>(Q, 4096);
+    srv->maxL = maxL;
     srv->slot_bytes = need;
     srv->slots.resize(nindex);
     for (int i = 0; i < nindex; ++i) {
@@ -265,8 +427,12 @@ int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_confi
         int rc = culsh_bt_create(srv->ht[i], &sl.bt);
         if (rc != CULSH_OK) return bail(rc);
         e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&sl.io_done);
         if (e != hipSuccess) return bail(fail(CULSH_E_DEVICE, "%s", hipGetErrorString(e)));
-        rc = bt_reserve(sl.bt, srv->Qcap, srv->Rcap, srv->Pcap);
+        sl.Qcap = (int)Q;
+        sl.Pcap = 32 * Q;
+        sl.Rcap = (int)std::min<long long>(Q, 4096);
+        rc = bt_reserve(sl.bt, sl.Qcap, sl.Rcap, sl.Pcap);
         if (rc == CULSH_OK) rc = bt_capture(sl.bt);
         if (rc != CULSH_OK) return bail(rc);
       }
@@ -289,27 +455,19 @@ void culsh_srv_destroy(culsh_srv* srv) { delete srv; }
 int culsh_srv_run(culsh_srv* srv, const culsh_srv_request* reqs, int n,
                   const float* const* d_queries, int* const* d_ids, float* const* d_dists,
                   double* latency_ms, culsh_srv_stats* stats) {
-  if (!srv || (!reqs && n > 0) || !d_queries || !d_ids || !d_dists)
-    return fail(CULSH_E_ARG, "invalid arguments");
-  if (n == 0) return CULSH_OK;
-  TRY(srv_check(srv, reqs, n));
-  std::vector<double> lat(n, 0.0);
-  double last = 0.0;
-  long long batches = 0, breqs = 0, deferred = 0;
-  if (srv->cfg.batching)
-    TRY(srv_run_batched(srv, reqs, n, d_queries, d_ids, d_dists, lat, &last, &batches, &breqs,
-                        &deferred));
-  else
-    TRY(srv_run_plain(srv, reqs, n, d_queries, d_ids, d_dists, lat, &last));
-  if (latency_ms) std::copy(lat.begin(), lat.end(), latency_ms);
-  if (stats) {
-    if (!srv->cfg.batching) {
-      batches = n;
-      breqs = n;
-    }
-    fill_stats(lat, reqs, n, last, batches, breqs, deferred, srv->slot_bytes, stats);
-  }
-  return CULSH_OK;
+  return srv_run(srv, reqs, n, d_queries, d_ids, d_dists, HostIO{}, latency_ms, stats);
+}
+
+int culsh_srv_run_host(culsh_srv* srv, const culsh_srv_request* reqs, int n,
+                       const float* const* h_queries, int* const* h_ids, float* const* h_dists,
+                       const float* const* d_queries, int* const* d_ids, float* const* d_dists,
+                       double* latency_ms, culsh_srv_stats* stats) {
+  if (!h_queries || !h_ids || !h_dists) return fail(CULSH_E_ARG, "null host arrays");
+  HostIO io;
+  io.q = h_queries;
+  io.ids = h_ids;
+  io.dists = h_dists;
+  return srv_run(srv, reqs, n, d_queries, d_ids, d_dists, io, latency_ms, stats);
 }
 
 }  // extern "C"

@@ -325,8 +325,20 @@ class Batch(_Base):
     def getRechecked(self):
         return int(self._L.culsh_bt_rechecked(self._h))
 
+    def profile(self, reqs, queries, ids, dists, stream=0):
+        """One eager run of the batch's kernels (no graph) with HIP events
+        between stages on `stream`; returns the stage times in ms
+        {probe, locality, candidates, rest} or None on failure."""
+        arr = _requests(reqs)
+        ms = (ctypes.c_float * 4)()
+        if not self._ok(self._L.culsh_bt_profile(self._h, arr, len(reqs), queries.data_ptr(),
+                                                 ids.data_ptr(), dists.data_ptr(),
+                                                 ctypes.c_void_p(stream), ms)):
+            return None
+        return dict(zip(("probe", "locality", "candidates", "rest"), list(ms)))
+
     def autotune(self, reqs, queries, ids, dists, reps=2):
-        """Returns (default ms, tuned ms); the tuned schedule stays set."""
+        """Returns (naive default ms, tuned ms); the tuned schedule stays set."""
         arr = _requests(reqs)
         a, b = ctypes.c_float(), ctypes.c_float()
         if not self._ok(self._L.culsh_bt_autotune(self._h, arr, len(reqs), queries.data_ptr(),
@@ -338,17 +350,19 @@ class Batch(_Base):
 
 class Server(_Base):
     """Serving form: a real-time request loop over device-resident indexes.
-    batching=True: continuous batching (slots of the batching form, admitted
-    by each slot's share of free HBM); batching=False: one request at a time
-    through the search path (the plain baseline)."""
+    batching=True: continuous batching (slots of the batching form, sized
+    from the free HBM; a request larger than a slot grows it when the HBM
+    free at that moment allows, unless grow=False, or runs in pieces);
+    batching=False: one request at a time through the search path (the plain
+    baseline)."""
 
     def __init__(self, indexes, batching=True, slots_per_index=2, hbm_fraction=0.5,
-                 max_batch_queries=0):
+                 max_batch_queries=0, grow=True):
         self._L = _lib.lib()
         self._hts = list(indexes)
         arr = (ctypes.c_void_p * len(self._hts))(*[h.handle.value for h in self._hts])
         cfg = SrvConfig(int(slots_per_index), int(bool(batching)), float(hbm_fraction),
-                        int(max_batch_queries))
+                        int(max_batch_queries), int(not grow))
         h = ctypes.c_void_p()
         if self._L.culsh_srv_create(arr, len(self._hts), ctypes.byref(cfg), ctypes.byref(h)) != 0:
             raise RuntimeError(_lib.last_error())
@@ -359,23 +373,33 @@ class Server(_Base):
             self._L.culsh_srv_destroy(self._h)
             self._h = ctypes.c_void_p()
 
-    def run(self, reqs, queries, ids, dists):
+    def run(self, reqs, queries, ids, dists, host=None):
         """reqs: [(arrival_s, index, Q, K, T, q_row, out_offset)] sorted by arrival;
-        queries / ids / dists: per index, device tensors.  Returns
-        (latencies_ms numpy array, stats dict) or None on failure."""
+        queries / ids / dists: per index, device tensors.  host: optional
+        (queries, ids, dists) per index as pinned CPU tensors: each batch then
+        uploads its query rows and downloads its results, and latencies
+        include both.  Returns (latencies_ms numpy array, stats dict) or None
+        on failure."""
         n = len(reqs)
         arr = (SrvRequest * n)()
         for i, (a, ix, Q, K, T, row, off) in enumerate(reqs):
             arr[i] = SrvRequest(float(a), int(ix), int(Q), int(K), int(T), int(row), int(off))
         k = len(self._hts)
-        qp = (ctypes.c_void_p * k)(*[t.data_ptr() for t in queries])
-        ip = (ctypes.c_void_p * k)(*[t.data_ptr() for t in ids])
-        dp = (ctypes.c_void_p * k)(*[t.data_ptr() for t in dists])
+
+        def ptrs(ts):
+            return (ctypes.c_void_p * k)(*[t.data_ptr() for t in ts])
+
         lat = np.zeros(n, np.float64)
         st = SrvStats()
-        ok = self._ok(self._L.culsh_srv_run(self._h, arr, n, qp, ip, dp,
-                                            lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                            ctypes.byref(st)))
-        if not ok:
+        latp = lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if host is None:
+            rc = self._L.culsh_srv_run(self._h, arr, n, ptrs(queries), ptrs(ids), ptrs(dists),
+                                       latp, ctypes.byref(st))
+        else:
+            hq, hi, hd = host
+            rc = self._L.culsh_srv_run_host(self._h, arr, n, ptrs(hq), ptrs(hi), ptrs(hd),
+                                            ptrs(queries), ptrs(ids), ptrs(dists), latp,
+                                            ctypes.byref(st))
+        if not self._ok(rc):
             return None
         return lat, {f: getattr(st, f) for f, _ in SrvStats._fields_}

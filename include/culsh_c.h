@@ -198,20 +198,34 @@ int culsh_bt_wait(culsh_bt* bt);
 int culsh_bt_last_ms(culsh_bt* bt, float* ms);
 /* Candidates re-evaluated in FP64 by the last batch. */
 unsigned long long culsh_bt_rechecked(culsh_bt* bt);
+/* Runs the batch once without the graph, the same kernels on `stream` with
+ * HIP events between its stages, and waits: stage_ms[0] = descriptor
+ * expansion + probe stage, [1] = locality order of the candidate stage,
+ * [2] = candidate stage (k_batch_cand; the fused schedule's one kernel),
+ * [3] = overflow kernel + header download.  Results are written as by
+ * culsh_bt_search_device. */
+int culsh_bt_profile(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
+                     int* d_ids, float* d_dists, void* stream, float* stage_ms);
 /* Coordinate search over the schedule on this batch (reps timed runs per
- * candidate schedule, best kept); leaves the best schedule set.  Reports the
- * default and tuned batch times (ms). */
+ * candidate schedule, best kept), starting from a fixed naive schedule
+ * (fused, 32x1 probe tile, 32 queries per item, 4 rows in flight, one
+ * workgroup per CU) and sweeping until a pass changes nothing; leaves the
+ * best schedule set.  Reports the naive and tuned batch times (ms). */
 int culsh_bt_autotune(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d_queries,
                       int* d_ids, float* d_dists, int reps, float* default_ms, float* tuned_ms);
 
 /* ---- Serving (continuous batching over device-resident indexes) ---------
- * No counterpart in the reference.  A request loop on the calling thread:
+ * No counterpart in the reference, whose search takes one query set per call
+ * and sizes its candidate batches from the free device memory
+ * (cuLSH_SearchTables.cu:283-311).  A request loop on the calling thread:
  * requests arrive at their arrival times, wait in a per-index queue, and are
- * admitted into the next batch of that index while the batch fits one slot's
- * share of HBM (the slots' buffers are sized once, from the free HBM when
- * the server is created).  Every slot holds one culsh_bt; batches of
- * different slots run concurrently on their own streams.  A request's
- * results equal a SearchTables search of it alone. */
+ * admitted into the next batch of that index while the batch fits a slot.
+ * Slots are sized at creation from the free HBM; a request larger than an
+ * empty slot grows the slot when the HBM free at that moment (times
+ * hbm_fraction) covers it, and otherwise runs in pieces over successive
+ * batches -- no request is rejected for its size.  Every slot holds one
+ * culsh_bt; batches of different slots run concurrently on their own
+ * streams.  A request's results equal a SearchTables search of it alone. */
 typedef struct culsh_srv culsh_srv;
 
 typedef struct {
@@ -230,6 +244,8 @@ typedef struct {
   long long max_batch_queries; /* per batch (0 = 4096); HBM may lower it.  The
                                   batch plans' fixed grids scale with it, so a
                                   smaller cap means less per-batch overhead */
+  int no_growth;          /* 1: slots keep their initial size; a request
+                             larger than a slot always runs in pieces */
 } culsh_srv_config;
 
 typedef struct {
@@ -239,8 +255,11 @@ typedef struct {
   double requests_per_s;
   long long batches, queries, requests;
   double mean_batch_requests;
-  long long slot_bytes;   /* HBM reserved per slot */
-  long long deferred;     /* admissions postponed for lack of slot memory */
+  long long slot_bytes;   /* HBM reserved per slot at creation */
+  long long deferred;     /* admissions postponed for lack of slot room */
+  long long split_requests; /* requests that ran in pieces (larger than a slot) */
+  long long grown_slots;  /* slot growths by the free HBM at admission */
+  int host_io;            /* 1: latencies include query upload + result download */
 } culsh_srv_stats;
 
 int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_config* cfg,
@@ -253,6 +272,17 @@ void culsh_srv_destroy(culsh_srv* srv);
 int culsh_srv_run(culsh_srv* srv, const culsh_srv_request* reqs, int n,
                   const float* const* d_queries, int* const* d_ids, float* const* d_dists,
                   double* latency_ms, culsh_srv_stats* stats);
+/* Same, with the queries in host memory and the results wanted there: each
+ * batch uploads its requests' query rows (h_queries[i] -> d_queries[i], same
+ * row offsets) on the slot's stream before its graph launch and downloads
+ * their results (d_ids[i] / d_dists[i] -> h_ids[i] / h_dists[i], same
+ * offsets) after it; a request completes when its results are in host
+ * memory.  Host arrays should be pinned (hipHostMalloc) for the copies to
+ * overlap. */
+int culsh_srv_run_host(culsh_srv* srv, const culsh_srv_request* reqs, int n,
+                       const float* const* h_queries, int* const* h_ids, float* const* h_dists,
+                       const float* const* d_queries, int* const* d_ids, float* const* d_dists,
+                       double* latency_ms, culsh_srv_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@ as the reference produces them)."""
 import numpy as np
 import pytest
 
+import oracle
 from culsh_amd import workload as wl
 from lshtest import assert_buckets_equal, assert_search_equal, run_gpu, run_oracle
 
@@ -334,3 +335,66 @@ def test_random_reset_statistics():
     assert st.search(X[:50], 50, X)
     # A data point used as a query finds itself at distance 0.
     assert (st.getKnnIds()[:, 0] == np.arange(50)).all()
+
+
+def test_precision_search_after_reindex_uses_the_new_dataset():
+    """One SearchTables in the precision form across re-indexing: index(X1),
+    then index(X2) of the same shape (its device copy usually lands at the
+    same address), then index_device on one tensor whose contents change in
+    place.  Every search must use the current dataset's row norms."""
+    import torch
+    from culsh_amd import HashTables, SearchTables
+    print("precision reindex", flush=True)
+    N, D, L, M, W, K, T = 6000, 24, 4, 6, 2.0, 10, 8
+    A, b = wl.projection(L, D, M, W, 601)
+    X1 = wl.mixture(N, D, 20, 0.2, 602)
+    X2 = wl.mixture(N, D, 20, 0.2, 603) * 1.5
+    Y = wl.queries_near(X2, 80, 0.05, 604)
+    ht = HashTables()
+    assert ht.reset(N, D, L, M, W, A, b)
+    assert ht.set_form(1)
+    st = SearchTables()
+    assert st.reset(ht, K, T)
+    assert st.set_form(1)
+    for X in (X1, X2, X1):
+        assert ht.index(X), ht.last_error
+        assert st.search(Y, Y.shape[0], X), st.last_error
+        o = oracle.Index(X, L, M, W, A, b).search(Y, X, K, T)
+        np.testing.assert_array_equal(st.getKnnIds(), o["ids"])
+        np.testing.assert_array_equal(st.getKnnDistances(), o["dists"])
+    dX = torch.from_numpy(X1.copy()).cuda()
+    ids = torch.empty((Y.shape[0], K), dtype=torch.int32, device="cuda")
+    d = torch.empty((Y.shape[0], K), dtype=torch.float32, device="cuda")
+    dY = torch.from_numpy(Y).cuda()
+    for X in (X1, X2):
+        dX.copy_(torch.from_numpy(X))
+        assert ht.index_device(dX), ht.last_error
+        assert st.search_device(dY, ids, d), st.last_error
+        assert st.wait(), st.last_error
+        o = oracle.Index(X, L, M, W, A, b).search(Y, X, K, T)
+        np.testing.assert_array_equal(ids.cpu().numpy(), o["ids"])
+        np.testing.assert_array_equal(d.cpu().numpy(), o["dists"])
+
+
+def test_search_dataset_upload_follows_the_host_pointer():
+    """After index_device, search(queries, Q, dataset) uploads the host
+    dataset once and reuses it while the same array is passed; another array
+    is uploaded again (distances follow the dataset passed, as the
+    reference's search does)."""
+    import torch
+    from culsh_amd import HashTables, SearchTables
+    print("search dataset upload", flush=True)
+    X, Y, A, b = _small(701, N=4000, Q=60)
+    ix = oracle.Index(X, 4, 6, 1.0, A, b)
+    dX = torch.from_numpy(X).cuda()
+    ht = HashTables()
+    assert ht.reset(X.shape[0], X.shape[1], 4, 6, 1.0, A, b)
+    assert ht.index_device(dX), ht.last_error
+    st = SearchTables()
+    assert st.reset(ht, 5, 8)
+    X2 = (X * 0.5).astype(np.float32)
+    for data in (X, X, X2, X):
+        assert st.search(Y, Y.shape[0], data), st.last_error
+        o = ix.search(Y, data, 5, 8)
+        np.testing.assert_array_equal(st.getKnnIds(), o["ids"])
+        np.testing.assert_array_equal(st.getKnnDistances(), o["dists"])

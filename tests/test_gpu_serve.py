@@ -96,13 +96,59 @@ def test_serve_admission_by_slot_memory():
     _check_alone(hts, reqs, Ys, ids, ds)
 
 
-def test_serve_rejects_oversized_request():
-    import torch
-    print("serve limits", flush=True)
-    hts, reqs, Ys, dY, outs = _setup(n=10, seed=321)
+def test_serve_runs_requests_larger_than_a_slot():
+    """Slots of 4 queries: a request of up to 100 queries either grows its
+    slot (the free HBM allows it) or, with growth off, runs in pieces over
+    several batches; results equal the request alone either way."""
+    print("serve oversized requests", flush=True)
+    hts, reqs, Ys, dY, outs = _setup(n=40, rate=20000.0, qmax=100, seed=321)
+    assert max(r[2] for r in reqs) > 4
+    for grow in (False, True):
+        srv = Server(hts, batching=True, slots_per_index=1, hbm_fraction=0.1,
+                     max_batch_queries=4, grow=grow)
+        ids, ds = _buffers(outs)
+        r = srv.run(reqs, dY, ids, ds)
+        assert r is not None, srv.last_error
+        lat, stats = r
+        print("  grow" if grow else "  pieces", stats, flush=True)
+        if grow:
+            assert stats["grown_slots"] > 0
+        else:
+            assert stats["split_requests"] > 0 and stats["grown_slots"] == 0
+        assert (lat > 0).all()
+        _check_alone(hts, reqs, Ys, ids, ds)
     srv = Server(hts, batching=True, slots_per_index=1, hbm_fraction=0.1, max_batch_queries=4)
     ids, ds = _buffers(outs)
-    big = [(0.0, 0, 5, 10, 16, 0, 0)]
-    assert srv.run(big, dY, ids, ds) is None
-    assert "capacity" in srv.last_error
-    assert srv.run([(0.0, 2, 1, 1, 1, 0, 0)], dY, ids, ds) is None
+    assert srv.run([(0.0, 2, 1, 1, 1, 0, 0)], dY, ids, ds) is None  # no index 2
+    assert "out of range" in srv.last_error
+
+
+def test_serve_run_after_a_failed_run():
+    """A run that fails while other slots hold batches (a request with more
+    than 2^20 candidates: every point in one bucket) leaves no slot busy; the
+    next, smaller run on the same server returns every result and latency."""
+    import torch
+    print("serve run after failure", flush=True)
+    hts, reqs, Ys, dY, outs = _setup(n=120, rate=1e9, qmax=40, seed=331)
+    rng = np.random.default_rng(7)
+    Xb = rng.standard_normal((300000, 4)).astype(np.float32)
+    bad = HashTables()
+    assert bad.reset(300000, 4, 4, 1, 1e6, np.zeros((4, 1, 4), np.float32),
+                     np.zeros((4, 1), np.float32))
+    assert bad.index(Xb)
+    srv = Server([hts[0], bad], batching=True, slots_per_index=3, hbm_fraction=0.1)
+    yb = torch.from_numpy(Xb[:8].copy()).cuda()
+    ib = torch.empty(64, dtype=torch.int32, device="cuda")
+    db = torch.empty(64, dtype=torch.float32, device="cuda")
+    ids, ds = _buffers(outs)
+    a_reqs = [r for r in reqs if r[1] == 0]
+    failing = a_reqs[:60] + [(a_reqs[59][0], 1, 2, 5, 1, 0, 0)] + a_reqs[60:]
+    assert srv.run(failing, [dY[0], yb], [ids[0], ib], [ds[0], db]) is None
+    assert "candidates" in srv.last_error
+    small = a_reqs[:10]
+    ids, ds = _buffers(outs)
+    r = srv.run(small, [dY[0], yb], [ids[0], ib], [ds[0], db])
+    assert r is not None, srv.last_error
+    lat, stats = r
+    assert stats["requests"] == 10 and lat.shape == (10,) and (lat > 0).all()
+    _check_alone([hts[0], bad], small, Ys, ids, ds)
