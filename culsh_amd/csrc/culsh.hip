@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1104,33 +1105,24 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
   return CULSH_OK;
 }
 
-// The device search pipeline on `s`: probe kernel, candidate kernel (and the
-// global-memory kernel for queries that overflow it).  Synchronises to check
-// the per-query capacity flags.
-int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
-               hipStream_t s, bool timed, bool wait = true) {
-  TRY(st_check_async(st, wait));
-  // A device search still running on another stream uses the same working
-  // buffers: order this one after it.
-  if (st->pending) HIP_TRY(hipStreamWaitEvent(s, st->evdone, 0));
-  // The plain form can finish without the host: its overflowed queries go
-  // to a fixed-size device kernel.  The precision form sizes its group
-  // working sets on the host.
-  const bool async = !wait && st->form == CULSH_FORM_PLAIN;
+// Working buffers of one device search.
+struct SearchBufs {
+  int* qb;                          // [L][Q*T] probe buckets
+  unsigned long long* cnu_qt;       // [Q*L]
+  unsigned long long *cnu_q, *cu_q; // [Q]
+  int* ov;                          // [Q] overflowed queries
+  int* nov;                         // their count (zeroed before the launch)
+  unsigned long long* cnt;          // FP64 recheck counter (zeroed)
+  bool order;                       // may run large batches in locality order
+};
+
+// Probe stage then candidate stage on `s` (events ev[0..2] around them when
+// `timed`).  The caller has zeroed *b.nov and *b.cnt on `s`.
+int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, int Q,
+                                int* d_ids, float* d_dists, hipStream_t s, const SearchBufs& b,
+                                bool timed) {
   culsh_ht* ht = st->ht;
   const int L = ht->L, M = ht->M, D = ht->D, T = st->T, K = st->K;
-  const long long QT = (long long)Q * T;
-  if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
-  TRY(st->dqb.ensure((size_t)L * QT));
-  TRY(st->dcnu_qt.ensure((size_t)Q * L));
-  TRY(st->dcnu_q.ensure(Q));
-  TRY(st->dcu_q.ensure(Q));
-  TRY(st->dov.ensure(Q));
-  TRY(st->dnov.ensure(1));
-  TRY(st->dcnt.ensure(1));
-  TRY(st->dmax.ensure(1));
-  HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
   if (timed) HIP_TRY(hipEventRecord(st->ev[0], s));
   const long long pairs = (long long)Q * L;
   const int lds8 = probe8_lds_bytes(D, M);
@@ -1142,7 +1134,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
                      (const float*)ht->dAd.p, ht->Jp, (const double*)ht->db64.p, (double)ht->W, \
                      (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
                      (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
-                     (const int16_t*)ht->bcodes.p, st->dqb.p, st->dcnu_qt.p)
+                     (const int16_t*)ht->bcodes.p, b.qb, b.cnu_qt)
 #define CULSH_PROBE8_M(JJ)   \
   if (M <= 8)                \
     CULSH_PROBE8(JJ, 1);     \
@@ -1175,7 +1167,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
                      (const double*)ht->dAt64.p, (const double*)ht->db64.p, (double)ht->W,   \
                      (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
                      (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
-                     (const int16_t*)ht->bcodes.p, st->dqb.p, st->dcnu_qt.p)
+                     (const int16_t*)ht->bcodes.p, b.qb, b.cnu_qt)
     if (T <= 64)
       CULSH_PROBE(1);
     else if (T <= 128)
@@ -1198,18 +1190,50 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
     // of it, so queries that share candidate rows share an L2.
     const uint32_t* order = nullptr;
     unsigned grid = (unsigned)Q;
-    if (Q >= kOrderMinQ) {
+    if (b.order && Q >= kOrderMinQ) {
       TRY(locality_order(st, dQ, Q, s, &order));
       grid = (unsigned)((Q + 7) / 8 * 8);
     }
     hipLaunchKernelGGL(k_candidates, dim3(grid), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
-                       (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
-                       (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
-                       d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p,
-                       st->dcnt.p, order);
+                       (const int*)b.qb, (const BucketRec*)ht->recs.p,
+                       (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt, d_ids,
+                       d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order);
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
   HIP_TRY(hipGetLastError());
+  return CULSH_OK;
+}
+
+// The device search pipeline on `s`: probe kernel, candidate kernel (and the
+// global-memory kernel for queries that overflow it).  Synchronises to check
+// the per-query capacity flags, unless `wait` is false in the plain form.
+int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
+               hipStream_t s, bool timed, bool wait = true) {
+  TRY(st_check_async(st, wait));
+  // A device search still running on another stream uses the same working
+  // buffers: order this one after it.
+  if (st->pending) HIP_TRY(hipStreamWaitEvent(s, st->evdone, 0));
+  // The plain form can finish without the host: its overflowed queries go
+  // to a fixed-size device kernel.  The precision form sizes its group
+  // working sets on the host.
+  const bool async = !wait && st->form == CULSH_FORM_PLAIN;
+  culsh_ht* ht = st->ht;
+  const int L = ht->L, D = ht->D, T = st->T, K = st->K;
+  const long long QT = (long long)Q * T;
+  if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
+  TRY(st->dqb.ensure((size_t)L * QT));
+  TRY(st->dcnu_qt.ensure((size_t)Q * L));
+  TRY(st->dcnu_q.ensure(Q));
+  TRY(st->dcu_q.ensure(Q));
+  TRY(st->dov.ensure(Q));
+  TRY(st->dnov.ensure(1));
+  TRY(st->dcnt.ensure(1));
+  TRY(st->dmax.ensure(1));
+  HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
+  SearchBufs b{st->dqb.p, st->dcnu_qt.p, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p,
+               st->dcnt.p, true};
+  TRY(launch_probe_and_candidates(st, dQ, dX, Q, d_ids, d_dists, s, b, timed));
   if (async) {
     TRY(st->dovfix.ensure((size_t)kOvAsyncGrid *
                           ((1ull << kOvAsyncHbits) + 2ull * kOvAsyncUcap)));
@@ -1400,6 +1424,131 @@ const unsigned long long* culsh_st_query_cu(culsh_st* st) {
 unsigned long long culsh_st_rechecked(culsh_st* st) { return st ? st->rechecked : 0; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free device search (culsh_st_search_device_noalloc)
+// ---------------------------------------------------------------------------
+// Working memory of the allocation-free search: one copy per device, held
+// until the process ends.  Host code never passes this name to a kernel or a
+// copy (on the host it is only a stand-in for the device array); it takes
+// the device address from hipGetSymbolAddress, once per device.
+static constexpr size_t kNoAllocBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Fixed overflow working sets of the allocation-free search: workgroups,
+// hash bits and the largest candidate list of one query (2^17 non-unique
+// candidates; a query with more sets the caller's status word).
+constexpr int kNaOvGrid = 4;
+constexpr int kNaOvHbits = 18;
+constexpr unsigned kNaOvUcap = 1u << 17;
+constexpr size_t kNaOvBytes = (size_t)kNaOvGrid * ((1ull << kNaOvHbits) + 2ull * kNaOvUcap) * 4;
+
+std::mutex g_na_mu;
+constexpr int kNaMaxDevices = 64;
+void* g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once).
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices)
+    return fail(CULSH_E_DEVICE, "device %d beyond the %d the library tracks", dev, kNaMaxDevices);
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CULSH_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_na_ptr[dev] = p;
+  }
+  *out = g_na_ptr[dev];
+  return CULSH_OK;
+}
+
+// Clears the counters of one allocation-free search and the caller's status.
+__global__ void k_na_clear(int* nov, unsigned long long* cnt, int* status) {
+  if (threadIdx.x == 0) {
+    *nov = 0;
+    *cnt = 0ull;
+    *status = 0;
+  }
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Byte offsets of the search buffers in the scratch array.
+struct NaLayout {
+  size_t ovfix, ctr, cnu_q, cu_q, ov, cnu_qt, qb, total;
+};
+
+NaLayout na_layout(long long Q, int L, int T) {
+  NaLayout o;
+  o.ovfix = 0;
+  o.ctr = align256(kNaOvBytes);
+  o.cnu_q = o.ctr + 256;
+  o.cu_q = align256(o.cnu_q + 8ull * Q);
+  o.ov = align256(o.cu_q + 8ull * Q);
+  o.cnu_qt = align256(o.ov + 4ull * Q);
+  o.qb = align256(o.cnu_qt + 8ull * Q * L);
+  o.total = o.qb + 4ull * Q * T * L;
+  return o;
+}
+
+}  // namespace
+
+extern "C" int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queries, int Q,
+                                              int* d_knn_ids, float* d_knn_dists, int* d_status,
+                                              void* stream) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  if (!st->ht)
+    return fail(CULSH_E_STATE,
+                "No indexing structure has been associated with searching structure...");
+  culsh_ht* ht = st->ht;
+  if (Q <= 0 || st->K <= 0 || st->T <= 0)
+    return fail(CULSH_E_ARG, "Q, K and T must be positive");
+  if (!ht->indexed || !ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
+  if (!d_queries || !d_knn_ids || !d_knn_dists || !d_status)
+    return fail(CULSH_E_ARG, "null device pointer");
+  if (st->form != CULSH_FORM_PLAIN)
+    return fail(CULSH_E_ARG, "the allocation-free search runs the plain form only");
+  const int L = ht->L, T = st->T, D = ht->D, K = st->K;
+  if (T > 1024) return fail(CULSH_E_LIMIT, "T = %d exceeds the supported maximum of 1024", T);
+  if ((long long)L * Q * T >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
+  const NaLayout lay = na_layout(Q, L, T);
+  if (lay.total > kNoAllocBytes) {
+    const NaLayout one = na_layout(1, L, T);
+    const long long per = (long long)(one.total - one.cnu_q);
+    return fail(CULSH_E_LIMIT,
+                "allocation-free search: Q = %d with L = %d, T = %d needs %zu bytes of working "
+                "memory; its fixed scratch holds %zu bytes (about %lld queries at this L and T)",
+                Q, L, T, lay.total, kNoAllocBytes,
+                (long long)((kNoAllocBytes - lay.cnu_q) / (size_t)(per + 64)));
+  }
+  void* basev = nullptr;
+  TRY(noalloc_scratch(&basev));
+  unsigned char* base = static_cast<unsigned char*>(basev);
+  hipStream_t s = (hipStream_t)stream;
+  int* nov = reinterpret_cast<int*>(base + lay.ctr);
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.ctr + 8);
+  hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(64), 0, s, nov, cnt, d_status);
+  SearchBufs b{reinterpret_cast<int*>(base + lay.qb),
+               reinterpret_cast<unsigned long long*>(base + lay.cnu_qt),
+               reinterpret_cast<unsigned long long*>(base + lay.cnu_q),
+               reinterpret_cast<unsigned long long*>(base + lay.cu_q),
+               reinterpret_cast<int*>(base + lay.ov), nov, cnt, false};
+  TRY(launch_probe_and_candidates(st, d_queries, ht->dX, Q, d_knn_ids, d_knn_dists, s, b, false));
+  hipLaunchKernelGGL(k_candidates_overflow, dim3(kNaOvGrid), dim3(256), 0, s, ht->dX, d_queries,
+                     Q, D, L, T, K, (const int*)b.qb, (const BucketRec*)ht->recs.p,
+                     (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt,
+                     d_knn_ids, d_knn_dists, b.cnu_q, b.cu_q, (const int*)b.ov,
+                     (const int*)b.nov, reinterpret_cast<uint32_t*>(base + lay.ovfix),
+                     kNaOvHbits, kNaOvUcap, b.cnt, d_status);
+  HIP_TRY(hipGetLastError());
+  return CULSH_OK;
+}
 
 // Batching form (culsh_bt_*): shares this translation unit's helpers and
 // index state.

@@ -198,6 +198,15 @@ class SearchTables(_Base):
                                                        ids.data_ptr(), dists.data_ptr(),
                                                        ctypes.c_void_p(stream)))
 
+    def search_device_noalloc(self, queries, ids, dists, status, stream=0):
+        """Allocation-free device search (plain form): like search_device, with
+        the working buffers in the library's fixed device array; `status` is a
+        device int32 tensor set non-zero when a query exceeded the fixed
+        candidate caps (its results are then not written)."""
+        return self._ok(self._L.culsh_st_search_device_noalloc(
+            self._h, queries.data_ptr(), int(queries.shape[0]), ids.data_ptr(), dists.data_ptr(),
+            status.data_ptr(), ctypes.c_void_p(stream)))
+
     def wait(self):
         """Waits for the last device search; False (and the message) if it or an
         earlier asynchronous search failed."""

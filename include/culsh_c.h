@@ -125,6 +125,23 @@ int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_k
 /* Waits for the last device search and reports its (or an earlier one's)
  * failure. */
 int culsh_st_wait(culsh_st* st);
+/* Allocation-free variant of culsh_st_search_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same kernels and results as culsh_st_search_device (plain form
+ * only; no locality reordering of large query sets, which changes no
+ * result), but every working buffer lives in one fixed 64 MiB device array
+ * of the library (one per GPU, kept until the process ends), so the call
+ * allocates nothing and never synchronises.  Limits: the working set of
+ * the call (about 20 + 8 L + 4 L T bytes per query, plus 8 MiB) must fit the
+ * array, else CULSH_E_LIMIT before any launch (c3's L = 32, T = 64: about
+ * 6,900 queries); a query with more than 2^17 non-unique candidates or more
+ * than 4096 candidates within the FP32 error bound of its K-th distance
+ * gets no results and sets *d_status (a device int the caller provides,
+ * cleared by the call) to non-zero.  All calls on one GPU share the array:
+ * calls that overlap on the device race (order them on one stream). */
+int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
+                                   float* d_knn_dists, int* d_status, void* stream);
 /* Distance form of search(): CULSH_FORM_PLAIN (default) or _PRECISION. */
 int culsh_st_set_form(culsh_st* st, int form);
 
