@@ -83,6 +83,7 @@ SIGNATURES = {
     "culsh_st_reset": (c_int, [c_vp, c_vp, c_int, c_int]),
     "culsh_st_search": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp]),
     "culsh_st_search_device": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp, c_vp]),
+    "culsh_st_set_pass_limit": (c_int, [c_vp, c_int]),
     "culsh_st_search_device_noalloc": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),
     "culsh_st_wait": (c_int, [c_vp]),
     "culsh_st_set_form": (c_int, [c_vp, c_int]),

@@ -435,6 +435,10 @@ struct CandBufs {
   uint32_t* sid;    // their ids
   unsigned scap;
   float* ys;        // query vector staging (kYcap floats, may alias hash), or null
+  // Optional [3] shader-clock totals of the query's phases: union (member
+  // expansion + dedup), FP32 distances, top-K (selection, FP64 recheck,
+  // sort, tie rule); one atomic add per phase per query, or null.
+  unsigned long long* phase = nullptr;
 };
 
 // Where one query's inputs and outputs live.  `slot` indexes its C_nu row
@@ -481,6 +485,15 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   const int K = io.K;
 
   __syncthreads();  // a previous query's readers of the shared counters are done
+  const bool stamp = B.phase != nullptr && tid == 0;
+  unsigned long long t_mark = stamp ? (unsigned long long)clock64() : 0ull;
+  auto phase_end = [&](int i) {
+    if (stamp) {
+      const unsigned long long now = (unsigned long long)clock64();
+      atomicAdd(&B.phase[i], now - t_mark);
+      t_mark = now;
+    }
+  };
   if (tid < 64) {
     unsigned long long c = 0;
     for (int t = tid; t < L; t += 64) c += cnu_qt[io.slot * L + t];
@@ -523,6 +536,7 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
     });
   }
   __syncthreads();
+  phase_end(0);
   if (s_ovf) return false;
   const int U = (int)s_count;
 
@@ -535,11 +549,16 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
     l2_fp32_rows<R>(X, B.ys, D, B.ulist, U, B.d32);
   else
     l2_fp32_rows<R>(X, y, D, B.ulist, U, B.d32);
+  if (B.phase) {  // uniform: every thread takes the same branch
+    __syncthreads();
+    phase_end(1);
+  }
   // Direct FP32: |d32 - d| <= (D + 3) u d; margin 2 (Higham ch. 3).
   const double rel = 2.0 * (double)(D + 8) * kU32d;
-  if (!finish_topk(X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd, B.sid,
-                   B.scap, io.oid, io.od, n_recheck))
-    return false;
+  const bool done = finish_topk(X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd,
+                                B.sid, B.scap, io.oid, io.od, n_recheck);
+  phase_end(2);
+  if (!done) return false;
   if (tid == 0) {
     cnu_q[io.slot] = cnu;
     cu_q[io.slot] = (unsigned long long)U;
@@ -561,7 +580,7 @@ __global__ __launch_bounds__(256, 3) void k_candidates(
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, int* __restrict__ overflow,
     int* __restrict__ n_overflow, unsigned long long* __restrict__ n_recheck,
-    const uint32_t* __restrict__ order) {
+    const uint32_t* __restrict__ order, unsigned long long* __restrict__ phase) {
   __shared__ uint32_t s_hash[kHcap];
   __shared__ uint32_t s_u[kUcap];
   static_assert(kUcap + 3 * kScap + kYcap <= kHcap, "post-union LDS layout");
@@ -575,6 +594,7 @@ __global__ __launch_bounds__(256, 3) void k_candidates(
   B.sid = s_hash + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_hash + kUcap + 3 * kScap);
+  B.phase = phase;
   int q = blockIdx.x;
   if (order) {
     const int per = (int)(gridDim.x >> 3);  // grid is a multiple of 8
@@ -608,7 +628,8 @@ __global__ __launch_bounds__(256) void k_candidates_global(
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, const int* __restrict__ list, int n_list,
     uint32_t* __restrict__ scratch, int hbits, unsigned ucap,
-    unsigned long long* __restrict__ n_recheck, int* __restrict__ err) {
+    unsigned long long* __restrict__ n_recheck, int* __restrict__ err,
+    unsigned long long* __restrict__ phase) {
   __shared__ double s_sd[kScapG];
   __shared__ uint32_t s_sid[kScapG];
   __shared__ float s_y[kYcap];
@@ -623,6 +644,7 @@ __global__ __launch_bounds__(256) void k_candidates_global(
   B.sid = s_sid;
   B.scap = kScapG;
   B.ys = s_y;
+  B.phase = phase;
   for (int i = blockIdx.x; i < n_list; i += gridDim.x) {
     if (!query_candidates(search_io(list[i], Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs,
                           sids, cnu_qt, cnu_q, cu_q, n_recheck, B)) {

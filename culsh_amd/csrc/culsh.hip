@@ -20,6 +20,7 @@
 
 #include "../../include/culsh_c.h"
 #include "batch_kernels.hpp"
+#include "canon.hpp"
 #include "common.hpp"
 #include "index_kernels.hpp"
 #include "philox.hpp"
@@ -155,7 +156,9 @@ struct culsh_ht {
   long long nslots = 0;
   unsigned long long rechecked = 0;
 
-  // Lexicographic numbering and host mirrors, built on demand.
+  // Lexicographic numbering (renum_ok) and host mirrors (canon), built on
+  // demand.
+  bool renum_ok = false;
   bool canon = false;
   DevBuf<int> renumber;  // global bucket -> table-local lexicographic index
   std::vector<unsigned> hB;
@@ -168,6 +171,7 @@ struct culsh_ht {
   long long B_total() const { return table_base.empty() ? 0 : (long long)table_base[L]; }
   void drop_index() {
     indexed = false;
+    renum_ok = false;
     canon = false;
     data_gen = next_data_gen();
     dX_own.release();
@@ -440,6 +444,7 @@ int index_resident(culsh_ht* ht, FILE* dbg) {
   TRY(project_dataset(ht, codes, dbg));
   TRY(build_buckets(ht, codes, dbg));
   ht->indexed = true;
+  ht->renum_ok = false;
   ht->canon = false;
   if (dbg)
     fprintf(dbg, "[HashTables::index]\t%d tables indexed, %lld buckets in total\n", ht->L,
@@ -447,68 +452,135 @@ int index_resident(culsh_ht* ht, FILE* dbg) {
   return CULSH_OK;
 }
 
-// Lexicographic bucket numbering per table (the reference's bucket order,
-// cuLSH_Indexing.cu:71-93) and host mirrors of the bucket arrays.
+// Lexicographic order of every table's buckets (global bucket indices,
+// table by table) into perm [B]: the reference's bucket order
+// (cuLSH_Indexing.cu:71-93), computed on the device by stable LSD passes
+// over keys that each pack as many code columns as fit 64 bits, last
+// columns first, then a pass on the table index.
+int lex_order(culsh_ht* ht, DevBuf<uint32_t>& perm) {
+  hipStream_t s = ht->stream;
+  const long long B = ht->B_total();
+  const int M = ht->M;
+  DevBuf<uint64_t> k0, k1;
+  DevBuf<uint32_t> p1, scr;
+  DevBuf<int> rng;
+  TRY(perm.exact(B));
+  TRY(p1.exact(B));
+  TRY(k0.exact(B));
+  TRY(k1.exact(B));
+  TRY(scr.exact(rs_scratch_elems(B) + 1));
+  TRY(rng.exact(2));
+  hipLaunchKernelGGL(k_lex_init, dim3(1), dim3(64), 0, s, rng.p);
+  const long long nc = B * M;
+  hipLaunchKernelGGL(k_code_range, dim3((unsigned)std::min<long long>(grid_for(nc), 4096)),
+                     dim3(256), 0, s, (const int16_t*)ht->bcodes.p, nc, rng.p);
+  int range[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(range, rng.p, sizeof(range), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const int w = std::max(1, ceil_log2_host((long long)range[1] - range[0] + 1));
+  const int per_key = std::max(1, 64 / w);
+  uint64_t *ka = k0.p, *kb = k1.p;
+  uint32_t *pa = perm.p, *pb = p1.p;
+  auto pass = [&](int bits) {
+    uint64_t* kk = ka;
+    uint32_t* vv = pa;
+    radix_sort_pairs(&kk, &vv, kb, pb, B, 0, (bits + 7) / 8 * 8, scr.p, s);
+    if (kk != ka) {
+      std::swap(ka, kb);
+      std::swap(pa, pb);
+    }
+  };
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(B)), dim3(256), 0, s, pa, B);
+  for (int hi = M; hi > 0; hi -= per_key) {
+    const int m0 = std::max(0, hi - per_key), ncols = hi - m0;
+    hipLaunchKernelGGL(k_lex_key, dim3(grid_for(B)), dim3(256), 0, s, (const uint32_t*)pa, B,
+                       (const int16_t*)ht->bcodes.p, M, m0, ncols, w, range[0], ka);
+    pass(ncols * w);
+  }
+  hipLaunchKernelGGL(k_lex_table_key, dim3(grid_for(B)), dim3(256), 0, s, (const uint32_t*)pa, B,
+                     (const BucketRec*)ht->recs.p, ht->tb, ka);
+  pass(ht->tb);
+  if (pa != perm.p) std::swap(perm.p, p1.p);
+  HIP_TRY(hipGetLastError());
+  return CULSH_OK;
+}
+
+// The table-local lexicographic index of every bucket (for the reference's
+// query-bucket numbering), on the device.
+int ensure_renumber(culsh_ht* ht) {
+  if (ht->renum_ok) return CULSH_OK;
+  if (!ht->indexed) return fail(CULSH_E_STATE, "no index: call index() or load() first");
+  const long long B = ht->B_total();
+  DevBuf<uint32_t> perm, tbase;
+  TRY(lex_order(ht, perm));
+  TRY(tbase.exact(ht->L + 1));
+  HIP_TRY(hipMemcpyAsync(tbase.p, ht->table_base.data(), (ht->L + 1) * 4, hipMemcpyHostToDevice,
+                         ht->stream));
+  TRY(ht->renumber.exact(std::max<long long>(B, 1)));
+  hipLaunchKernelGGL(k_lex_renumber, dim3(grid_for(B)), dim3(256), 0, ht->stream,
+                     (const uint32_t*)perm.p, B, (const BucketRec*)ht->recs.p, ht->tb,
+                     (const uint32_t*)tbase.p, ht->renumber.p);
+  HIP_TRY(hipStreamSynchronize(ht->stream));
+  HIP_TRY(hipGetLastError());
+  ht->renum_ok = true;
+  return CULSH_OK;
+}
+
+// Lexicographic numbering and the host mirrors of the bucket arrays that the
+// reference's getters and save() return: built on the device in the
+// reference's layout, then copied to host memory.
 int ensure_canonical(culsh_ht* ht) {
   if (ht->canon) return CULSH_OK;
   if (!ht->indexed) return fail(CULSH_E_STATE, "no index: call index() or load() first");
+  hipStream_t s = ht->stream;
   const long long N = ht->N, L = ht->L, M = ht->M, B = ht->B_total();
-  std::vector<int16_t> codes((size_t)B * M);
-  std::vector<BucketRec> recs(B);
-  std::vector<uint32_t> ids((size_t)L * N);
-  HIP_TRY(hipStreamSynchronize(ht->stream));
-  HIP_TRY(hipMemcpy(codes.data(), ht->bcodes.p, codes.size() * 2, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(recs.data(), ht->recs.p, B * sizeof(BucketRec), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(ids.data(), ht->sids.p, ids.size() * 4, hipMemcpyDeviceToHost));
-  std::vector<int> renum(B);
+  DevBuf<uint32_t> perm, tbase, sz, gst, lst, tot, scr;
+  DevBuf<float> codes;
+  TRY(lex_order(ht, perm));
+  TRY(tbase.exact(L + 1));
+  TRY(sz.exact(B));
+  TRY(gst.exact(B));
+  TRY(lst.exact(B));
+  TRY(tot.exact((size_t)L * N));
+  TRY(codes.exact((size_t)B * M));
+  TRY(scr.exact(scan_scratch_elems(B) + 1));
+  HIP_TRY(hipMemcpyAsync(tbase.p, ht->table_base.data(), (L + 1) * 4, hipMemcpyHostToDevice, s));
+  TRY(ht->renumber.exact(std::max<long long>(B, 1)));
+  const uint32_t* pp = perm.p;
+  const BucketRec* rp = ht->recs.p;
+  hipLaunchKernelGGL(k_lex_renumber, dim3(grid_for(B)), dim3(256), 0, s, pp, B, rp, ht->tb,
+                     (const uint32_t*)tbase.p, ht->renumber.p);
+  hipLaunchKernelGGL(k_lex_sizes, dim3(grid_for(B)), dim3(256), 0, s, pp, B, rp, sz.p);
+  scan_exclusive(sz.p, gst.p, B, scr.p, s);
+  hipLaunchKernelGGL(k_lex_starts, dim3(grid_for(B)), dim3(256), 0, s, pp, B, rp, ht->tb, N,
+                     (const uint32_t*)gst.p, lst.p);
+  hipLaunchKernelGGL(k_lex_members, dim3(grid_for(B)), dim3(256), 0, s, pp, B, rp,
+                     (const uint32_t*)ht->sids.p, (const uint32_t*)gst.p, tot.p);
+  hipLaunchKernelGGL(k_lex_codes, dim3(grid_for(B * M)), dim3(256), 0, s, pp, B,
+                     (const int16_t*)ht->bcodes.p, (int)M, rp, ht->tb, (const uint32_t*)tbase.p,
+                     codes.p);
+  HIP_TRY(hipGetLastError());
   ht->hB.assign(L, 0);
   ht->hTotal.assign(L, {});
   ht->hStart.assign(L, {});
   ht->hSize.assign(L, {});
   ht->hCodes.assign(L, {});
-  auto work = [&](int t) {
-    const long long b0 = ht->table_base[t], b1 = ht->table_base[t + 1], Bt = b1 - b0;
-    std::vector<uint32_t> order(Bt);
-    for (long long i = 0; i < Bt; ++i) order[i] = (uint32_t)(b0 + i);
-    const int16_t* cp = codes.data();
-    std::sort(order.begin(), order.end(), [cp, M](uint32_t a, uint32_t c) {
-      const int16_t* x = cp + (size_t)a * M;
-      const int16_t* y = cp + (size_t)c * M;
-      for (int m = 0; m < M; ++m)
-        if (x[m] != y[m]) return x[m] < y[m];
-      return false;
-    });
+  for (int t = 0; t < L; ++t) {
+    const long long b0 = ht->table_base[t], Bt = (long long)ht->table_base[t + 1] - b0;
     ht->hB[t] = (unsigned)Bt;
-    auto& tot = ht->hTotal[t];
-    auto& st = ht->hStart[t];
-    auto& sz = ht->hSize[t];
-    auto& cd = ht->hCodes[t];
-    tot.resize(N);
-    st.resize(Bt);
-    sz.resize(Bt);
-    cd.resize((size_t)Bt * M);
-    unsigned pos = 0;
-    for (long long i = 0; i < Bt; ++i) {
-      const uint32_t k = order[i];
-      renum[k] = (int)i;
-      const uint32_t a = recs[k].start, e = a + recs[k].size;
-      st[i] = pos;
-      sz[i] = e - a;
-      for (uint32_t r = a; r < e; ++r) tot[pos++] = ids[r];
-      for (int m = 0; m < M; ++m) cd[(size_t)m * Bt + i] = (float)cp[(size_t)k * M + m];
-    }
-  };
-  {
-    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (int w = 0; w < nth; ++w)
-      pool.emplace_back([&, w] {
-        for (int t = w; t < L; t += nth) work(t);
-      });
-    for (auto& th : pool) th.join();
+    ht->hTotal[t].resize(N);
+    ht->hStart[t].resize(Bt);
+    ht->hSize[t].resize(Bt);
+    ht->hCodes[t].resize((size_t)Bt * M);
+    HIP_TRY(hipMemcpyAsync(ht->hTotal[t].data(), tot.p + (size_t)t * N, N * 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ht->hStart[t].data(), lst.p + b0, Bt * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ht->hSize[t].data(), sz.p + b0, Bt * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ht->hCodes[t].data(), codes.p + (size_t)b0 * M, (size_t)Bt * M * 4,
+                           hipMemcpyDeviceToHost, s));
   }
-  TRY(ht->renumber.exact(std::max<long long>(B, 1)));
-  HIP_TRY(hipMemcpy(ht->renumber.p, renum.data(), B * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipStreamSynchronize(s));
+  ht->renum_ok = true;
   ht->canon = true;
   return CULSH_OK;
 }
@@ -784,6 +856,7 @@ int index_from_buckets(culsh_ht* ht, FILE* dbg) {
   HIP_TRY(hipStreamSynchronize(ht->stream));
   HIP_TRY(hipGetLastError());
   ht->indexed = true;
+  ht->renum_ok = true;
   ht->canon = true;
   return CULSH_OK;
 }
@@ -881,7 +954,12 @@ struct culsh_st {
   int K = 0, T = 1, Q = 0;
   int form = CULSH_FORM_PLAIN;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Host search(): per-phase shader-clock totals of the candidate stage
+  // (union, FP32 distances, top-K), gathered while `stamp` is set.
+  DevBuf<unsigned long long> dphase;
+  bool stamp = false;
+  int max_pass = 0;  // queries per pass cap (0: from the free HBM)
   DevBuf<float> dQ, dX_alt, ddists;
   DevBuf<int> dqb, dids, dov, dqb_local;
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt, dmax;
@@ -1006,7 +1084,8 @@ int run_overflow(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_i
                      (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
                      (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
                      d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p, nov,
-                     st->dscratch.p, hbits, ucap, st->dcnt.p, st->dnov.p);
+                     st->dscratch.p, hbits, ucap, st->dcnt.p, st->dnov.p,
+                     st->stamp ? st->dphase.p : nullptr);
   HIP_TRY(hipGetLastError());
   int err = 0;
   HIP_TRY(hipMemcpyAsync(&err, st->dnov.p, 4, hipMemcpyDeviceToHost, s));
@@ -1114,6 +1193,7 @@ struct SearchBufs {
   int* nov;                         // their count (zeroed before the launch)
   unsigned long long* cnt;          // FP64 recheck counter (zeroed)
   bool order;                       // may run large batches in locality order
+  unsigned long long* phase;        // [3] phase clocks of the candidate stage, or null
 };
 
 // Probe stage then candidate stage on `s` (events ev[0..2] around them when
@@ -1184,6 +1264,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[1], s));
   if (st->form == CULSH_FORM_PRECISION) {
+    if (timed) HIP_TRY(hipEventRecord(st->ev[3], s));
     TRY(run_group_candidates(st, dQ, dX, Q, d_ids, d_dists, s));
   } else {
     // Large batches run in locality order, each XCD taking a contiguous run
@@ -1194,10 +1275,11 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
       TRY(locality_order(st, dQ, Q, s, &order));
       grid = (unsigned)((Q + 7) / 8 * 8);
     }
+    if (timed) HIP_TRY(hipEventRecord(st->ev[3], s));
     hipLaunchKernelGGL(k_candidates, dim3(grid), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
                        (const int*)b.qb, (const BucketRec*)ht->recs.p,
                        (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt, d_ids,
-                       d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order);
+                       d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order, b.phase);
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
   HIP_TRY(hipGetLastError());
@@ -1232,7 +1314,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
   SearchBufs b{st->dqb.p, st->dcnu_qt.p, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p,
-               st->dcnt.p, true};
+               st->dcnt.p, true, st->stamp ? st->dphase.p : nullptr};
   TRY(launch_probe_and_candidates(st, dQ, dX, Q, d_ids, d_dists, s, b, timed));
   if (async) {
     TRY(st->dovfix.ensure((size_t)kOvAsyncGrid *
@@ -1299,13 +1381,44 @@ static int st_check(culsh_st* st, int Q) {
   return st_setup(st);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Queries per pass of a search (cuLSH_SearchTables.cu:283-311 sizes its
+// batches from the free device memory the same way): the probe buffers
+// address L*Q*T slots with 31-bit offsets, and a pass's working set must
+// fit half the HBM free at this moment (plus what the object already holds,
+// which the pass reuses).  `host`: the pass also stages queries, results and
+// lexicographic query buckets.
+int search_pass(culsh_st* st, int Q, bool host, int* out) {
+  culsh_ht* ht = st->ht;
+  const long long L = ht->L, T = st->T, K = st->K, D = ht->D;
+  long long q = std::min<long long>(Q, ((1ll << 31) - 1) / (L * T));
+  const long long per = 4 * L * T + 8 * L + 16 + 4 + 48 + (host ? 4 * L * T + 4 * D + 8 * K : 0);
+  size_t freeb = 0, totalb = 0;
+  HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+  long long held = (long long)(st->dqb.n * 4 + st->dcnu_qt.n * 8 + st->dcnu_q.n * 16);
+  if (host) held += (long long)(st->dqb_local.n * 4 + st->dQ.n * 4 + st->dids.n * 4 + st->ddists.n * 4);
+  q = std::min<long long>(q, ((long long)(freeb / 2) + held) / per);
+  if (st->max_pass > 0) q = std::min<long long>(q, st->max_pass);
+  if (q < 1)
+    return fail(CULSH_E_MEMORY, "free HBM (%zu bytes) holds no query of this search", freeb);
+  *out = (int)q;
+  return CULSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* dataset, FILE* dbg) {
   TRY(st_check(st, Q));
   if (!queries || !dataset) return fail(CULSH_E_ARG, "null queries or dataset");
   culsh_ht* ht = st->ht;
   const int D = ht->D, K = st->K, T = st->T, L = ht->L;
   st->clear_results();
-  TRY(ensure_canonical(ht));
+  TRY(ensure_renumber(ht));
   const float* dX = ht->dX;
   if (!dX || dataset != ht->hX_indexed) {
     // A dataset other than the resident one (an index built from a device
@@ -1324,48 +1437,102 @@ int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* data
     }
     dX = st->dX_alt.p;
   }
-  TRY(st->dQ.ensure((size_t)Q * D));
-  TRY(st->dids.ensure((size_t)Q * K));
-  TRY(st->ddists.ensure((size_t)Q * K));
-  HIP_TRY(hipMemcpyAsync(st->dQ.p, queries, (size_t)Q * D * 4, hipMemcpyHostToDevice, st->stream));
+  int pass = 0;
+  TRY(search_pass(st, Q, true, &pass));
   if (dbg)
-    fprintf(dbg, "[SearchTables::search]\tPerforming searching. (Q, K, T) = (%d, %d, %d)\n", Q, K,
-            T);
-  TRY(run_search(st, st->dQ.p, dX, Q, st->dids.p, st->ddists.p, st->stream, true));
+    fprintf(dbg, "[SearchTables::search]\tPerforming searching. (Q, K, T) = (%d, %d, %d), %d "
+                 "queries per pass\n", Q, K, T, pass);
   const long long QT = (long long)Q * T;
-  TRY(st->dqb_local.ensure((size_t)L * QT));
-  hipLaunchKernelGGL(k_local_buckets, dim3(grid_for(L * QT)), dim3(256), 0, st->stream,
-                     (const int*)st->dqb.p, QT, L, (const int*)ht->renumber.p, st->dqb_local.p);
-  st->Q = Q;
   st->ids.resize((size_t)Q * K);
   st->dists.resize((size_t)Q * K);
   st->cnu.resize(Q);
   st->cu.resize(Q);
   st->qbuckets.assign(L, std::vector<int>(QT));
-  HIP_TRY(hipMemcpyAsync(st->ids.data(), st->dids.p, (size_t)Q * K * 4, hipMemcpyDeviceToHost, st->stream));
-  HIP_TRY(hipMemcpyAsync(st->dists.data(), st->ddists.p, (size_t)Q * K * 4, hipMemcpyDeviceToHost, st->stream));
-  HIP_TRY(hipMemcpyAsync(st->cnu.data(), st->dcnu_q.p, (size_t)Q * 8, hipMemcpyDeviceToHost, st->stream));
-  HIP_TRY(hipMemcpyAsync(st->cu.data(), st->dcu_q.p, (size_t)Q * 8, hipMemcpyDeviceToHost, st->stream));
-  for (int t = 0; t < L; ++t)
-    HIP_TRY(hipMemcpyAsync(st->qbuckets[t].data(), st->dqb_local.p + (size_t)t * QT, QT * 4,
+  TRY(st->dphase.ensure(3));
+  double t_buckets = 0, t_sort = 0, t_cand = 0;
+  unsigned long long cyc[3] = {0, 0, 0}, nrc_all = 0;
+  for (int q0 = 0; q0 < Q; q0 += pass) {
+    const int qn = std::min(pass, Q - q0);
+    const long long qnT = (long long)qn * T;
+    TRY(st->dQ.ensure((size_t)qn * D));
+    TRY(st->dids.ensure((size_t)qn * K));
+    TRY(st->ddists.ensure((size_t)qn * K));
+    HIP_TRY(hipMemcpyAsync(st->dQ.p, queries + (size_t)q0 * D, (size_t)qn * D * 4,
+                           hipMemcpyHostToDevice, st->stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st->stream, st->dphase.p, 3);
+    st->stamp = true;
+    const int rc = run_search(st, st->dQ.p, dX, qn, st->dids.p, st->ddists.p, st->stream, true);
+    st->stamp = false;
+    TRY(rc);
+    HIP_TRY(hipEventRecord(st->ev[4], st->stream));
+    TRY(st->dqb_local.ensure((size_t)L * qnT));
+    hipLaunchKernelGGL(k_local_buckets, dim3(grid_for(L * qnT)), dim3(256), 0, st->stream,
+                       (const int*)st->dqb.p, qnT, L, (const int*)ht->renumber.p,
+                       st->dqb_local.p);
+    HIP_TRY(hipMemcpyAsync(st->ids.data() + (size_t)q0 * K, st->dids.p, (size_t)qn * K * 4,
                            hipMemcpyDeviceToHost, st->stream));
-  unsigned long long nrc = 0;
-  HIP_TRY(hipMemcpyAsync(&nrc, st->dcnt.p, 8, hipMemcpyDeviceToHost, st->stream));
-  HIP_TRY(hipStreamSynchronize(st->stream));
-  st->rechecked = nrc;
+    HIP_TRY(hipMemcpyAsync(st->dists.data() + (size_t)q0 * K, st->ddists.p, (size_t)qn * K * 4,
+                           hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipMemcpyAsync(st->cnu.data() + q0, st->dcnu_q.p, (size_t)qn * 8,
+                           hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipMemcpyAsync(st->cu.data() + q0, st->dcu_q.p, (size_t)qn * 8, hipMemcpyDeviceToHost,
+                           st->stream));
+    for (int t = 0; t < L; ++t)
+      HIP_TRY(hipMemcpyAsync(st->qbuckets[t].data() + (size_t)q0 * T,
+                             st->dqb_local.p + (size_t)t * qnT, qnT * 4, hipMemcpyDeviceToHost,
+                             st->stream));
+    unsigned long long nrc = 0, pc[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(&nrc, st->dcnt.p, 8, hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipMemcpyAsync(pc, st->dphase.p, 24, hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    nrc_all += nrc;
+    for (int i = 0; i < 3; ++i) cyc[i] += pc[i];
+    float a = 0.f, b = 0.f, c = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, st->ev[0], st->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, st->ev[1], st->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&c, st->ev[3], st->ev[4]));
+    t_buckets += a;
+    t_sort += b;
+    t_cand += c;
+  }
+  st->Q = Q;
+  st->rechecked = nrc_all;
   for (int q = 0; q < Q; ++q) {
     st->tot_nu += st->cnu[q];
     st->tot_u += st->cu[q];
   }
-  float ms1 = 0.f, ms2 = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms1, st->ev[0], st->ev[1]));
-  HIP_TRY(hipEventElapsedTime(&ms2, st->ev[1], st->ev[2]));
+  // cuLSH_SearchTables.cu:342-362,410,440 times its steps separately; here
+  // the candidate stage is one fused pass, whose measured time (candidate
+  // kernels, overflow kernel included) is split by the shader-clock shares
+  // of its phases: union (collect; the dedup happens in the same hash
+  // insert, so unique stays 0), FP32 distances, top-K (knn).  `sort` is the
+  // locality ordering of large query sets.
+  const double ctot = (double)cyc[0] + (double)cyc[1] + (double)cyc[2];
+  const double f0 = ctot > 0 ? cyc[0] / ctot : 0.0, f1 = ctot > 0 ? cyc[1] / ctot : 0.0,
+               f2 = ctot > 0 ? cyc[2] / ctot : 0.0;
   st->times = culsh_times{};
-  st->times.buckets = ms1 / 1000.f;
-  st->times.distances = ms2 / 1000.f;
+  st->times.buckets = (float)(t_buckets / 1000.0);
+  st->times.sort = (float)(t_sort / 1000.0);
+  st->times.collect = (float)(t_cand * f0 / 1000.0);
+  st->times.unique = 0.f;
+  st->times.distances = (float)(t_cand * f1 / 1000.0);
+  st->times.knn = (float)(t_cand * f2 / 1000.0);
+  if (st->form == CULSH_FORM_PRECISION) {
+    // The grouped kernel is not phase-stamped: its whole time is reported
+    // as distance time.
+    st->times.collect = st->times.knn = 0.f;
+    st->times.distances = (float)(t_cand / 1000.0);
+  }
   if (dbg)
     fprintf(dbg, "Non-unique: %llu, unique: %llu, selectivity: %.2f%%\n", st->tot_nu, st->tot_u,
             culsh_st_selectivity(st) * 100.0);
+  return CULSH_OK;
+}
+
+int culsh_st_set_pass_limit(culsh_st* st, int max_queries) {
+  if (!st) return fail(CULSH_E_ARG, "null SearchTables");
+  if (max_queries < 0) return fail(CULSH_E_ARG, "max_queries must be >= 0");
+  st->max_pass = max_queries;
   return CULSH_OK;
 }
 
@@ -1374,8 +1541,15 @@ int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_k
   TRY(st_check(st, Q));
   if (!d_queries || !d_knn_ids || !d_knn_dists) return fail(CULSH_E_ARG, "null device pointer");
   if (!st->ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
-  return run_search(st, d_queries, st->ht->dX, Q, d_knn_ids, d_knn_dists, (hipStream_t)stream,
-                    true, false);
+  int pass = 0;
+  TRY(search_pass(st, Q, false, &pass));
+  const int D = st->ht->D, K = st->K;
+  for (int q0 = 0; q0 < Q; q0 += pass) {
+    const int qn = std::min(pass, Q - q0);
+    TRY(run_search(st, d_queries + (size_t)q0 * D, st->ht->dX, qn, d_knn_ids + (size_t)q0 * K,
+                   d_knn_dists + (size_t)q0 * K, (hipStream_t)stream, true, false));
+  }
+  return CULSH_OK;
 }
 
 int culsh_st_wait(culsh_st* st) {
@@ -1538,7 +1712,7 @@ extern "C" int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queri
                reinterpret_cast<unsigned long long*>(base + lay.cnu_qt),
                reinterpret_cast<unsigned long long*>(base + lay.cnu_q),
                reinterpret_cast<unsigned long long*>(base + lay.cu_q),
-               reinterpret_cast<int*>(base + lay.ov), nov, cnt, false};
+               reinterpret_cast<int*>(base + lay.ov), nov, cnt, false, nullptr};
   TRY(launch_probe_and_candidates(st, d_queries, ht->dX, Q, d_knn_ids, d_knn_dists, s, b, false));
   hipLaunchKernelGGL(k_candidates_overflow, dim3(kNaOvGrid), dim3(256), 0, s, ht->dX, d_queries,
                      Q, D, L, T, K, (const int*)b.qb, (const BucketRec*)ht->recs.p,

@@ -154,6 +154,41 @@ int io_download(const HostIO& io, const culsh_srv_request& q, int q0, int nq,
   return CULSH_OK;
 }
 
+// The uploads and downloads of one batch's pieces (all of index i), merged
+// into contiguous ranges: requests that arrived in sequence usually hold
+// adjacent query rows and adjacent result slots.
+int io_batch(const HostIO& io, const culsh_srv_request* reqs,
+             const std::vector<culsh_srv::Piece>& pieces, int i, int D, const float* const* dQ,
+             int* const* d_ids, float* const* d_dists, hipStream_t s, bool upload) {
+  long long a0 = -1, n = 0;  // current range (rows, or result elements)
+  auto flush = [&]() -> int {
+    if (n <= 0) return CULSH_OK;
+    if (upload) {
+      HIP_TRY(hipMemcpyAsync(const_cast<float*>(dQ[i]) + a0 * D, io.q[i] + a0 * D,
+                             (size_t)n * D * 4, hipMemcpyHostToDevice, s));
+    } else {
+      HIP_TRY(hipMemcpyAsync(io.ids[i] + a0, d_ids[i] + a0, (size_t)n * 4,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(io.dists[i] + a0, d_dists[i] + a0, (size_t)n * 4,
+                             hipMemcpyDeviceToHost, s));
+    }
+    return CULSH_OK;
+  };
+  for (const auto& pc : pieces) {
+    const culsh_srv_request& q = reqs[pc.r];
+    const long long b = upload ? q.q_row + pc.q0 : q.out_offset + (long long)pc.q0 * q.K;
+    const long long m = upload ? pc.nq : (long long)pc.nq * q.K;
+    if (n > 0 && b == a0 + n) {
+      n += m;
+    } else {
+      TRY(flush());
+      a0 = b;
+      n = m;
+    }
+  }
+  return flush();
+}
+
 // One request at a time through the search path (the plain baseline).
 int srv_run_plain(culsh_srv* srv, const culsh_srv_request* reqs, int n,
                   const float* const* dQ, int* const* d_ids, float* const* d_dists,
@@ -323,13 +358,13 @@ int srv_run_batched_loop(culsh_srv* srv, const culsh_srv_request* reqs, int n,
         if (breq.empty()) continue;
         sl.busy = true;  // from here on the slot holds work of this run
         if (io.on())
-          for (const auto& pc : sl.pieces)
-            TRY(io_upload(io, reqs[pc.r], srv->ht[i]->D, pc.q0, pc.nq, dQ, sl.stream));
+          TRY(io_batch(io, reqs, sl.pieces, i, srv->ht[i]->D, dQ, d_ids, d_dists, sl.stream,
+                       true));
         TRY(bt_launch(sl.bt, breq.data(), (int)breq.size(), dQ[i], d_ids[i], d_dists[i],
                       sl.stream));
         if (io.on()) {
-          for (const auto& pc : sl.pieces)
-            TRY(io_download(io, reqs[pc.r], pc.q0, pc.nq, d_ids, d_dists, sl.stream));
+          TRY(io_batch(io, reqs, sl.pieces, i, srv->ht[i]->D, dQ, d_ids, d_dists, sl.stream,
+                       false));
           HIP_TRY(hipEventRecord(sl.io_done, sl.stream));
         }
         ++c.batches;

@@ -198,6 +198,10 @@ class SearchTables(_Base):
                                                        ids.data_ptr(), dists.data_ptr(),
                                                        ctypes.c_void_p(stream)))
 
+    def set_pass_limit(self, max_queries):
+        """Caps the queries per pass (0: sized from the free device memory)."""
+        return self._ok(self._L.culsh_st_set_pass_limit(self._h, int(max_queries)))
+
     def search_device_noalloc(self, queries, ids, dists, status, stream=0):
         """Allocation-free device search (plain form): like search_device, with
         the working buffers in the library's fixed device array; `status` is a

@@ -29,9 +29,14 @@ extern "C" {
 typedef struct culsh_ht culsh_ht; /* cuLSH::HashTables  (cuLSH_HashTables.cu:51-108) */
 typedef struct culsh_st culsh_st; /* cuLSH::SearchTables (cuLSH_SearchTables.cu:42-107) */
 
-/* cuLSH::SearchTables::timesStruct (cuLSH_SearchTables.cu:44-52), seconds.
- * collect = gathering bucket members, sort = ordering, unique = dedup; the
- * device pipeline fuses collect/sort/unique, reported under collect. */
+/* cuLSH::SearchTables::timesStruct (cuLSH_SearchTables.cu:44-52), seconds,
+ * of the last culsh_st_search: buckets = projection, multiprobe and lookup
+ * (probe kernel); sort = locality ordering of large query sets; collect,
+ * distances, knn = the fused candidate stage's time (overflow kernel
+ * included) split by the measured shader-clock shares of its phases --
+ * member gathering with hash dedup (so unique, a separate step in the
+ * reference, is 0), FP32 distances, and selection + FP64 recheck + top-K.
+ * The precision form's grouped kernel reports its whole time as distances. */
 typedef struct {
   float buckets, collect, sort, unique, distances, knn;
 } culsh_times;
@@ -108,9 +113,17 @@ void culsh_st_destroy(culsh_st* st);
 /* SearchTables::reset(ht, K, T) (cuLSH_SearchTables.cu:127-139). */
 int culsh_st_reset(culsh_st* st, culsh_ht* ht, int K, int T);
 /* SearchTables::search(queries, Q, dataset, dbg) (cuLSH_SearchTables.cu:197-463):
- * host [D x Q] queries; dataset must be the indexed one (when it is the same
- * host pointer index() was given, the resident copy is used). */
+ * host [D x Q] queries; distances are taken against `dataset` (when it is
+ * the host pointer index() was given, the resident copy is used; another
+ * host array is uploaded once and reused while the same pointer is passed
+ * for the same index).  Like the reference (cuLSH_SearchTables.cu:283-311),
+ * the queries run in passes sized from the free device memory at the call;
+ * a pass also holds at most (2^31 - 1) / (L T) queries. */
 int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* dataset, FILE* dbg);
+/* Caps the queries per pass of culsh_st_search / culsh_st_search_device
+ * (0, the default: the free device memory decides).  No counterpart in the
+ * reference; results do not depend on it. */
+int culsh_st_set_pass_limit(culsh_st* st, int max_queries);
 /* Device-resident search: queries, ids and dists are device pointers
  * ([D x Q], [K x Q], [K x Q]); runs on `stream` (a hipStream_t, 0 = null
  * stream) and returns without synchronising.  Query buckets and candidate

@@ -208,3 +208,30 @@ def test_serving_with_host_io_equals_device_resident(c5):
     assert r is not None, srv2.last_error
     for i in range(2):
         np.testing.assert_array_equal(hI[i].numpy(), dev[0][i])
+
+
+def test_config3_search_beyond_31_bit_probe_slots(c3):
+    """1,100,000 queries at L = 32, T = 64 address more than 2^31 probe slots:
+    the device search runs them in two passes (cuLSH_SearchTables.cu:283-311
+    batches by memory the same way); the first and last 500 queries equal
+    their search alone."""
+    import torch
+    from culsh_amd import SearchTables
+    cfg = c3.cfg
+    K, T = 10, cfg.T
+    Q = 1_100_000
+    reps = (Q + cfg.Q - 1) // cfg.Q
+    dY = torch.from_numpy(c3.Y).cuda().repeat(reps, 1)[:Q].contiguous()
+    st = SearchTables()
+    assert st.reset(c3.ht, K, T)
+    ids = torch.empty((Q, K), dtype=torch.int32, device="cuda")
+    d = torch.empty((Q, K), dtype=torch.float32, device="cuda")
+    assert st.search_device(dY, ids, d), st.last_error
+    assert st.wait(), st.last_error
+    for sl in (slice(0, 500), slice(Q - 500, Q)):
+        i2 = torch.empty((500, K), dtype=torch.int32, device="cuda")
+        d2 = torch.empty((500, K), dtype=torch.float32, device="cuda")
+        assert st.search_device(dY[sl].contiguous(), i2, d2), st.last_error
+        assert st.wait(), st.last_error
+        np.testing.assert_array_equal(ids[sl].cpu().numpy(), i2.cpu().numpy())
+        np.testing.assert_array_equal(d[sl].cpu().numpy(), d2.cpu().numpy())

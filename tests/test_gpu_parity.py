@@ -398,3 +398,34 @@ def test_search_dataset_upload_follows_the_host_pointer():
         o = ix.search(Y, data, 5, 8)
         np.testing.assert_array_equal(st.getKnnIds(), o["ids"])
         np.testing.assert_array_equal(st.getKnnDistances(), o["dists"])
+
+
+def test_search_in_passes_equals_one_pass():
+    """Queries split into passes (here forced to 7 queries per pass) give the
+    one-pass results: ids, distances, candidate counts and query buckets,
+    through the host and the device APIs; timesStruct reports every phase."""
+    import torch
+    from culsh_amd import SearchTables
+    print("search in passes", flush=True)
+    X, Y, A, b = _small(801, N=6000, Q=101)
+    o = run_oracle(X, Y, A, b, 4, 6, 1.0, 7, 12)
+    g = run_gpu(X, Y, A, b, 4, 6, 1.0, 7, 12)
+    st = SearchTables()
+    assert st.reset(g["ht"], 7, 12)
+    assert st.set_pass_limit(7)
+    assert st.search(Y, Y.shape[0], X), st.last_error
+    cnu, cu = st.getQueryCandidates()
+    gp = dict(ids=st.getKnnIds(), dists=st.getKnnDistances(), cnu=cnu, cu=cu,
+              query_buckets=np.stack([st.getQueryBuckets(t) for t in range(4)]))
+    assert_search_equal(gp, o)
+    t = st.getTimes()
+    print("  times", t, flush=True)
+    assert t["buckets"] > 0 and t["collect"] > 0 and t["distances"] > 0 and t["knn"] > 0
+    assert t["unique"] == 0 and t["sort"] >= 0
+    dY = torch.from_numpy(Y).cuda()
+    ids = torch.empty((Y.shape[0], 7), dtype=torch.int32, device="cuda")
+    d = torch.empty((Y.shape[0], 7), dtype=torch.float32, device="cuda")
+    assert st.search_device(dY, ids, d), st.last_error
+    assert st.wait(), st.last_error
+    np.testing.assert_array_equal(ids.cpu().numpy(), o["ids"])
+    np.testing.assert_array_equal(d.cpu().numpy(), o["dists"])
