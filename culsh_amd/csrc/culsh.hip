@@ -22,6 +22,7 @@
 #include "batch_kernels.hpp"
 #include "canon.hpp"
 #include "common.hpp"
+#include "index_file.hpp"
 #include "index_kernels.hpp"
 #include "philox.hpp"
 #include "search_kernels.hpp"
@@ -204,11 +205,8 @@ bool params_ok(int N, int D, int L, int M, float W) {
 }
 
 int check_limits(int N, int D, int L, int M) {
-  if (M > 64) return fail(CULSH_E_LIMIT, "M = %d exceeds the supported maximum of 64", M);
-  if ((long long)L * N >= (1ll << 31))
-    return fail(CULSH_E_LIMIT, "L*N = %lld exceeds 2^31 - 1", (long long)L * N);
-  if ((long long)L * M >= (1 << 24)) return fail(CULSH_E_LIMIT, "L*M exceeds 2^24");
-  if (D > (1 << 20)) return fail(CULSH_E_LIMIT, "D exceeds 2^20");
+  const std::string m = culsh_file::limits_message(N, D, L, M);
+  if (!m.empty()) return fail(CULSH_E_LIMIT, "%s", m.c_str());
   return CULSH_OK;
 }
 
@@ -862,62 +860,26 @@ int index_from_buckets(culsh_ht* ht, FILE* dbg) {
 }
 
 int load_file(culsh_ht* ht, FILE* fp, const char* filename, FILE* dbg) {
-  int N = 0, D = 0, L = 0, M = 0;
-  float W = 0.f;
-  auto rd = [fp](void* p, size_t sz, size_t n) { return fread(p, sz, n, fp) == n; };
-  if (!rd(&N, 4, 1) || !rd(&D, 4, 1) || !rd(&L, 4, 1) || !rd(&M, 4, 1) || !rd(&W, 4, 1))
-    return fail(CULSH_E_IO, "\"%s\": truncated header", filename);
+  // The whole file is read and checked on the host first.
+  culsh_file::IndexFile f;
+  std::string err;
+  const int rc = culsh_file::read_index_file(fp, filename, f, err);
+  if (rc != CULSH_OK) return fail(rc, "%s", err.c_str());
   if (dbg)
     fprintf(dbg,
             "[HashTables::load]\tIndexing structure loaded from file \"%s\", (N, D, L, M, W) = "
             "(%d, %d, %d, %d, %.2f)\n",
-            filename, N, D, L, M, W);
-  TRY(set_params(ht, N, D, L, M, W));
-  ht->A.resize((size_t)L * D * M);
-  ht->b.resize((size_t)L * M);
-  for (int t = 0; t < L; ++t)
-    if (!rd(ht->A.data() + (size_t)t * D * M, 4, (size_t)D * M) ||
-        !rd(ht->b.data() + (size_t)t * M, 4, M))
-      return fail(CULSH_E_IO, "\"%s\": truncated projection of table %d", filename, t);
+            filename, f.N, f.D, f.L, f.M, f.W);
+  TRY(set_params(ht, f.N, f.D, f.L, f.M, f.W));
+  ht->A = std::move(f.A);
+  ht->b = std::move(f.b);
   TRY(upload_params(ht));
   ht->has_params = true;
-  ht->hB.assign(L, 0);
-  if (!rd(ht->hB.data(), 4, L)) return fail(CULSH_E_IO, "\"%s\": truncated bucket counts", filename);
-  ht->hTotal.assign(L, {});
-  ht->hStart.assign(L, {});
-  ht->hSize.assign(L, {});
-  ht->hCodes.assign(L, {});
-  for (int t = 0; t < L; ++t) {
-    const size_t B = ht->hB[t];
-    if (B == 0 || B > (size_t)N)
-      return fail(CULSH_E_IO, "\"%s\": table %d has %zu buckets for N = %d", filename, t, B, N);
-    ht->hTotal[t].resize(N);
-    ht->hStart[t].resize(B);
-    ht->hSize[t].resize(B);
-    ht->hCodes[t].resize(B * M);
-    if (!rd(ht->hTotal[t].data(), 4, N) || !rd(ht->hStart[t].data(), 4, B) ||
-        !rd(ht->hSize[t].data(), 4, B) || !rd(ht->hCodes[t].data(), 4, B * M))
-      return fail(CULSH_E_IO, "\"%s\": truncated buckets of table %d", filename, t);
-    // Consistency: sizes tile [0, N) in order and every id appears once.
-    uint64_t pos = 0;
-    for (size_t k = 0; k < B; ++k) {
-      if (ht->hStart[t][k] != pos || ht->hSize[t][k] == 0)
-        return fail(CULSH_E_IO, "\"%s\": table %d bucket %zu has an inconsistent start/size",
-                    filename, t, k);
-      pos += ht->hSize[t][k];
-    }
-    if (pos != (uint64_t)N)
-      return fail(CULSH_E_IO, "\"%s\": table %d bucket sizes sum to %llu, not N", filename, t,
-                  (unsigned long long)pos);
-    std::vector<char> seen(N, 0);
-    for (int i = 0; i < N; ++i) {
-      const uint32_t id = ht->hTotal[t][i];
-      if (id >= (uint32_t)N || seen[id])
-        return fail(CULSH_E_IO, "\"%s\": table %d lists id %u twice or out of range", filename, t,
-                    id);
-      seen[id] = 1;
-    }
-  }
+  ht->hB = std::move(f.B);
+  ht->hTotal = std::move(f.total);
+  ht->hStart = std::move(f.start);
+  ht->hSize = std::move(f.size);
+  ht->hCodes = std::move(f.codes);
   return index_from_buckets(ht, dbg);
 }
 

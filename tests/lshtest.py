@@ -1,5 +1,7 @@
 """Shared helpers for the parity tests: run the HIP path (through the C ABI)
 and the oracle on the same inputs and compare them."""
+import struct
+
 import numpy as np
 
 import oracle
@@ -66,3 +68,21 @@ def recall_at(ids, true_ids, true_d, X, Y, k=10):
             if i >= 0 and oracle.dist(Y[q], X[i]) <= dk:
                 hit += 1
     return hit / (Q * k)
+
+
+def write_reference_format(path, ix, A, b, N, D, L, M, W):
+    """cuLSH_HashTables.cu:415-433: int N, D, L, M; float W; per table A_t, b_t;
+    unsigned B[L]; per table totalIndices[N], startingIndices[B], sizes[B],
+    codes[B*M] (column-major)."""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iiiif", N, D, L, M, W))
+        for t in range(L):
+            f.write(np.ascontiguousarray(A[t], np.float32).tobytes())   # (M, D) == [D x M] col-major
+            f.write(np.ascontiguousarray(b[t], np.float32).tobytes())
+        f.write(np.array([ix.buckets(t) for t in range(L)], np.uint32).tobytes())
+        for t in range(L):
+            tot, start, size, codes = ix.table(t)
+            f.write(tot.astype(np.uint32).tobytes())
+            f.write(start.astype(np.uint32).tobytes())
+            f.write(size.astype(np.uint32).tobytes())
+            f.write(np.ascontiguousarray(codes.T, np.float32).tobytes())

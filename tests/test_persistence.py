@@ -14,26 +14,9 @@ import pytest
 
 from culsh_amd import HashTables, SearchTables, _lib
 from culsh_amd import workload as wl
+from lshtest import write_reference_format
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-
-
-def write_reference_format(path, ix, A, b, N, D, L, M, W):
-    """cuLSH_HashTables.cu:415-433: int N, D, L, M; float W; per table A_t, b_t;
-    unsigned B[L]; per table totalIndices[N], startingIndices[B], sizes[B],
-    codes[B*M] (column-major)."""
-    with open(path, "wb") as f:
-        f.write(struct.pack("<iiiif", N, D, L, M, W))
-        for t in range(L):
-            f.write(np.ascontiguousarray(A[t], np.float32).tobytes())   # (M, D) == [D x M] col-major
-            f.write(np.ascontiguousarray(b[t], np.float32).tobytes())
-        f.write(np.array([ix.buckets(t) for t in range(L)], np.uint32).tobytes())
-        for t in range(L):
-            tot, start, size, codes = ix.table(t)
-            f.write(tot.astype(np.uint32).tobytes())
-            f.write(start.astype(np.uint32).tobytes())
-            f.write(size.astype(np.uint32).tobytes())
-            f.write(np.ascontiguousarray(codes.T, np.float32).tobytes())
 
 
 def test_load_missing_file_is_io_error(tmp_path):
