@@ -12,6 +12,7 @@
 #pragma once
 #include <sys/types.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -74,7 +75,7 @@ inline int read_index_file(FILE* fp, const char* name, IndexFile& f, std::string
   f.M = hdr[3];
   const long long N = f.N, D = f.D, L = f.L, M = f.M;
   if (!(N > 0 && D > 0 && L > 0 && M > 0 && f.W > 0.0f))
-    return bad(CULSH_E_ARG,
+    return bad(CULSH_E_ARG, "%s",
                "Projection parameters must be positive! Have you reset the index structure "
                "properly???");
   const std::string lim = limits_message(N, D, L, M);
