@@ -129,6 +129,21 @@ def _recall(X, Y, ids, n):
             "ground_truth": "exact FP64 brute force (oracle), tie-aware"}
 
 
+def _index_twice(ht, dX):
+    """Builds the index twice: the first build of a process also pays the
+    one-time load of the library's GPU code at its first kernel launch; the
+    second is the steady-state build time.  Returns (first_ms, second_ms)."""
+    import torch
+    out = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assert ht.index_device(dX), ht.last_error
+        torch.cuda.synchronize()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out[0], out[1]
+
+
 class C3Index:
     """The c3 workload's data and its device-resident index (built once)."""
 
@@ -146,12 +161,9 @@ class C3Index:
         if form != "plain":
             assert self.ht.set_form(FORM_PRECISION), self.ht.last_error
         self.form = FORM_PLAIN if form == "plain" else FORM_PRECISION
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        assert self.ht.index_device(self.dX), self.ht.last_error
-        torch.cuda.synchronize()
-        self.index_ms = (time.perf_counter() - t0) * 1e3
-        log(f"c3 index {self.index_ms:.1f} ms")
+        self.index_first_ms, self.index_ms = _index_twice(self.ht, self.dX)
+        log(f"c3 index {self.index_ms:.1f} ms (first build in the process "
+            f"{self.index_first_ms:.1f} ms)")
         self._oracle = None
 
     def oracle(self):
@@ -204,7 +216,8 @@ def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps):
     assert np.array_equal(st.getKnnIds(), gpu_ids)
     cnu, cu = st.getQueryCandidates()
     probe_bytes, cand_bytes = _search_bytes(D, L, T, K, cnu, cu)
-    cand_kernel = "k_candidates" if form == FORM_PLAIN else "k_group_candidates"
+    from culsh_amd import FORM_PRECISION_GROUPED
+    cand_kernel = "k_group_candidates" if form == FORM_PRECISION_GROUPED else "k_candidates"
     out = {
         "value": round(Q * steps / elapsed, 1), "unit": "queries/s",
         "ms_per_step": round(elapsed / steps * 1e3, 4), "steps": steps,
@@ -215,7 +228,7 @@ def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps):
         "roofline": _roofline({"k_query_probe8": (probe_ms / steps, probe_bytes),
                                cand_kernel: (cand_ms / steps, cand_bytes)},
                               cand_kernel if cand_ms >= probe_ms else "k_query_probe8",
-                              f"{cfg.name.split('-')[0]}/{'plain' if form == FORM_PLAIN else 'precision'}"),
+                              f"{cfg.name.split('-')[0]}/{args.form}"),
     }
     return out, gpu_ids
 
@@ -225,23 +238,21 @@ def bench_search(args):
     import numpy as np
     import torch
 
-    from culsh_amd import FORM_PLAIN, FORM_PRECISION, HashTables
+    from culsh_amd import FORM_PLAIN, FORM_PRECISION, FORM_PRECISION_GROUPED, HashTables
     from culsh_amd import workload as wl
     cfg = wl.CONFIGS[args.config]
-    form = FORM_PLAIN if args.form == "plain" else FORM_PRECISION
+    form = {"plain": FORM_PLAIN, "precision": FORM_PRECISION,
+            "grouped": FORM_PRECISION_GROUPED}[args.form]
     X, Y, A, b = wl.make_data(cfg)
     N, D = X.shape
     ht = HashTables()
     assert ht.reset(N, D, cfg.L, cfg.M, cfg.W, A, b), ht.last_error
     if form != FORM_PLAIN:
-        assert ht.set_form(form), ht.last_error
+        assert ht.set_form(FORM_PRECISION), ht.last_error  # BF16x3 index projection
     dX = torch.from_numpy(X).cuda()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    assert ht.index_device(dX), ht.last_error
-    torch.cuda.synchronize()
-    index_ms = (time.perf_counter() - t0) * 1e3
+    index_first_ms, index_ms = _index_twice(ht, dX)
     sub, gpu_ids = bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, args.steps)
+    sub["index_first_ms"] = round(index_first_ms, 2)
     out = _base(args, sub["value"], "queries/s", sub["ms_per_step"] * 1e-3 * args.steps,
                 args.steps, cfg.describe(),
                 {"form": args.form, "N": N, "D": D, "L": cfg.L, "M": cfg.M, "W": cfg.W,
@@ -491,12 +502,9 @@ def serve_indexes():
         dX = torch.from_numpy(X).cuda()
         ht = HashTables()
         assert ht.reset(cfg.N, cfg.D, cfg.L, cfg.M, cfg.W, A, b), ht.last_error
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        assert ht.index_device(dX), ht.last_error
-        torch.cuda.synchronize()
+        _, index_ms = _index_twice(ht, dX)
         hts.append(ht)
-        data.append((cfg, X, A, b, dX, (time.perf_counter() - t0) * 1e3))
+        data.append((cfg, X, A, b, dX, index_ms))
     return hts, data
 
 
@@ -626,6 +634,7 @@ def bench_all(args):
     from culsh_amd import FORM_PLAIN
     c3s, c3_ids = bench_search_cfg(args, c3.cfg, c3.X, c3.Y, c3.A, c3.b, c3.ht, c3.index_ms,
                                    FORM_PLAIN, max(3, args.steps // 2))
+    c3s["index_first_ms"] = round(c3.index_first_ms, 2)
     log(f"c3 plain {c3s['value']:.0f} q/s")
     # c4 batching (headline).
     b = bench_batch_on(args, c3, args.steps)
@@ -658,7 +667,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--form", default="plain", choices=["plain", "precision"])
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "grouped"],
+                    help="c1-c3: plain (FP32 projection), precision (BF16x3 index projection), "
+                         "grouped (BF16x3 + grouped norm-expansion distances)")
     ap.add_argument("--cpu-queries", type=int, default=3000,
                     help="queries in the oracle's timed CPU sample (0 = skip)")
     ap.add_argument("--recall-queries", type=int, default=100)

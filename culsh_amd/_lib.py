@@ -120,6 +120,7 @@ SIGNATURES = {
 
 FORM_PLAIN = 0
 FORM_PRECISION = 1
+FORM_PRECISION_GROUPED = 2
 
 
 def lib():

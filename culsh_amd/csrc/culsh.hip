@@ -138,6 +138,8 @@ struct culsh_ht {
   DevBuf<float> dbj;      // [Jp]
   DevBuf<float> dnaj;     // [Jp]      upper bound of ||a_j||_2
   DevBuf<double> dAt64;   // [L][D][M]
+  DevBuf<uint16_t> dAhT, dAlT;  // [Jp][Dp] bf16 hi / lo parts of column j over d
+  int Dp = 0;                   // D rounded up to 32
   DevBuf<double> db64;    // [L][M]
   DevBuf<uint64_t> dseeds;
   uint64_t hash_seed = 0x243F6A8885A308D3ull;
@@ -211,6 +213,20 @@ int check_limits(int N, int D, int L, int M) {
 }
 
 // Uploads the projection parameters in the layouts the kernels read.
+// float -> bf16 bits, round to nearest even, and the split x ~ hi + lo used
+// by the precision form's projection (|x - hi - lo| <= 2^-16 |x|).
+uint16_t host_f2bf(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+float host_bf2f(uint16_t h) {
+  const uint32_t b = (uint32_t)h << 16;
+  float x;
+  std::memcpy(&x, &b, 4);
+  return x;
+}
+
 int upload_params(culsh_ht* ht) {
   TRY(ht_stream(ht));
   const int D = ht->D, L = ht->L, M = ht->M;
@@ -223,6 +239,9 @@ int upload_params(culsh_ht* ht) {
       naj(Jp, 0.f);
   std::vector<double> At64((size_t)L * D * M), b64((size_t)L * M);
   std::vector<uint64_t> seeds(L);
+  ht->Dp = (D + kPjKc - 1) / kPjKc * kPjKc;
+  const int Dp = ht->Dp;
+  std::vector<uint16_t> AhT((size_t)Jp * Dp, 0), AlT((size_t)Jp * Dp, 0);
   for (int t = 0; t < L; ++t) {
     const float* At = ht->A.data() + (size_t)t * D * M;
     for (int m = 0; m < M; ++m) {
@@ -232,6 +251,9 @@ int upload_params(culsh_ht* ht) {
         const float a = At[(size_t)m * D + d];
         Ad[(size_t)d * Jp + j] = a;
         Aorig[(size_t)j * D + d] = a;
+        const uint16_t hi = host_f2bf(a);
+        AhT[(size_t)j * Dp + d] = hi;
+        AlT[(size_t)j * Dp + d] = host_f2bf(a - host_bf2f(hi));
         At64[((size_t)t * D + d) * M + m] = (double)a;
         nrm += (double)a * (double)a;
       }
@@ -248,6 +270,10 @@ int upload_params(culsh_ht* ht) {
   TRY(ht->dAt64.exact(At64.size()));
   TRY(ht->db64.exact(b64.size()));
   TRY(ht->dseeds.exact(L));
+  TRY(ht->dAhT.exact(AhT.size()));
+  TRY(ht->dAlT.exact(AlT.size()));
+  HIP_TRY(hipMemcpy(ht->dAhT.p, AhT.data(), AhT.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ht->dAlT.p, AlT.data(), AlT.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ht->dAd.p, Ad.data(), Ad.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ht->dAorig.p, Aorig.data(), Aorig.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ht->dbj.p, bj.data(), Jp * 4, hipMemcpyHostToDevice));
@@ -286,9 +312,11 @@ int build_directory(culsh_ht* ht) {
   ht->shift = 64 - ht->tb - ht->P;
   ht->nslots = 1ll << (ht->tb + ht->P);
   TRY(ht->dir.exact(ht->nslots + 1));
-  hipLaunchKernelGGL(k_directory, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, ht->stream,
-                     (const BucketRec*)ht->recs.p, ht->B_total(), ht->shift, ht->nslots,
-                     ht->dir.p);
+  hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, ht->stream,
+                     ht->dir.p, ht->nslots + 1);
+  hipLaunchKernelGGL(k_directory, dim3(grid_for(std::max<long long>(ht->B_total(), 1))),
+                     dim3(256), 0, ht->stream, (const BucketRec*)ht->recs.p, ht->B_total(),
+                     ht->shift, ht->nslots, ht->dir.p);
   return CULSH_OK;
 }
 
@@ -309,6 +337,8 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
   TRY(excl.ensure(n_all));
   TRY(scratch.ensure(std::max(rs_scratch_elems(n_all), scan_scratch_elems(n_all)) + 1));
   TRY(dflags.ensure(1));
+  DevBuf<uint32_t> bases;
+  TRY(bases.ensure(L + 1));
   for (int attempt = 0;; ++attempt) {
     if (attempt > 0) {
       // 64-bit hash collision between two distinct codes: new seeds.
@@ -329,32 +359,34 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
       std::swap(k0.p, k1.p);
       std::swap(v0.p, v1.p);
     }
-    HIP_TRY(hipMemsetAsync(dflags.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_heads, dim3(grid_for(n_all)), dim3(256), 0, s, (const uint64_t*)k0.p,
-                       (const uint32_t*)v0.p, n_all, N, (int)M, ht->tb, codes.p, flag.p,
-                       dflags.p);
+                       n_all, flag.p);
+    scan_exclusive(flag.p, excl.p, n_all, scratch.p, s);
+    hipLaunchKernelGGL(k_table_bases, dim3(1), dim3(std::max(64, (int)((L + 64) / 64 * 64))), 0,
+                       s, (const uint32_t*)flag.p, (const uint32_t*)excl.p, N, (int)L, bases.p);
+    ht->table_base.assign(L + 1, 0);
+    HIP_TRY(hipMemcpyAsync(ht->table_base.data(), bases.p, (L + 1) * 4, hipMemcpyDeviceToHost,
+                           s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const long long B = ht->table_base[L];
+    TRY(ht->recs.exact(B));
+    TRY(ht->bcodes.exact(B * M));
+    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n_all)), dim3(256), 0, s,
+                       (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
+                       (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->recs.p,
+                       ht->bcodes.p);
+    hipLaunchKernelGGL(k_rec_sizes, dim3(grid_for(B)), dim3(256), 0, s, ht->recs.p, B, n_all);
+    HIP_TRY(hipMemsetAsync(dflags.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_verify_members, dim3(grid_for(n_all)), dim3(256), 0, s,
+                       (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
+                       (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p,
+                       (const int16_t*)ht->bcodes.p, dflags.p);
     int collision = 0;
     HIP_TRY(hipMemcpyAsync(&collision, dflags.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (!collision) break;
     if (dbg) fprintf(dbg, "[HashTables::index]\tbucket key collision, reseeding\n");
   }
-  scan_exclusive(flag.p, excl.p, n_all, scratch.p, s);
-  DevBuf<uint32_t> bases;
-  TRY(bases.ensure(L + 1));
-  hipLaunchKernelGGL(k_table_bases, dim3(1), dim3(std::max(64, (int)((L + 64) / 64 * 64))), 0, s,
-                     (const uint32_t*)flag.p, (const uint32_t*)excl.p, N, (int)L, bases.p);
-  ht->table_base.assign(L + 1, 0);
-  HIP_TRY(hipMemcpyAsync(ht->table_base.data(), bases.p, (L + 1) * 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  const long long B = ht->table_base[L];
-  TRY(ht->recs.exact(B));
-  TRY(ht->bcodes.exact(B * M));
-  hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n_all)), dim3(256), 0, s,
-                     (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
-                     (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->recs.p,
-                     ht->bcodes.p);
-  hipLaunchKernelGGL(k_rec_sizes, dim3(grid_for(B)), dim3(256), 0, s, ht->recs.p, B, n_all);
   TRY(build_directory(ht));
   // The sorted member ids become the index's id array.
   ht->sids.release();
@@ -393,9 +425,9 @@ int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
   if (grid >= (1ll << 31)) return fail(CULSH_E_LIMIT, "projection grid too large");
   if (ht->form == CULSH_FORM_PRECISION)
     hipLaunchKernelGGL(k_project_bf16x3, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
-                       (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
-                       (const float*)ht->dnaj.p, (const float*)nx.p, ht->W, codes.p, list.p,
-                       cnt.p, seg, err.p);
+                       (const uint16_t*)ht->dAhT.p, (const uint16_t*)ht->dAlT.p, ht->Dp, Jp, J,
+                       M, (const float*)ht->dbj.p, (const float*)ht->dnaj.p, (const float*)nx.p,
+                       ht->W, codes.p, list.p, cnt.p, seg, err.p);
   else
     hipLaunchKernelGGL(k_project_f32, dim3((unsigned)grid), dim3(256), 0, s, ht->dX, N, D,
                        (const float*)ht->dAd.p, Jp, J, M, (const float*)ht->dbj.p,
@@ -1225,7 +1257,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 #undef CULSH_PROBE
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[1], s));
-  if (st->form == CULSH_FORM_PRECISION) {
+  if (st->form == CULSH_FORM_PRECISION_GROUPED) {
     if (timed) HIP_TRY(hipEventRecord(st->ev[3], s));
     TRY(run_group_candidates(st, dQ, dX, Q, d_ids, d_dists, s));
   } else {
@@ -1250,17 +1282,18 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 
 // The device search pipeline on `s`: probe kernel, candidate kernel (and the
 // global-memory kernel for queries that overflow it).  Synchronises to check
-// the per-query capacity flags, unless `wait` is false in the plain form.
+// the per-query capacity flags, unless `wait` is false and the distances are
+// direct (plain and precision forms).
 int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
                hipStream_t s, bool timed, bool wait = true) {
   TRY(st_check_async(st, wait));
   // A device search still running on another stream uses the same working
   // buffers: order this one after it.
   if (st->pending) HIP_TRY(hipStreamWaitEvent(s, st->evdone, 0));
-  // The plain form can finish without the host: its overflowed queries go
-  // to a fixed-size device kernel.  The precision form sizes its group
+  // Direct distances can finish without the host: overflowed queries go
+  // to a fixed-size device kernel.  The grouped form sizes its group
   // working sets on the host.
-  const bool async = !wait && st->form == CULSH_FORM_PLAIN;
+  const bool async = !wait && st->form != CULSH_FORM_PRECISION_GROUPED;
   culsh_ht* ht = st->ht;
   const int L = ht->L, D = ht->D, T = st->T, K = st->K;
   const long long QT = (long long)Q * T;
@@ -1324,7 +1357,8 @@ int culsh_st_reset(culsh_st* st, culsh_ht* ht, int K, int T) {
 
 int culsh_st_set_form(culsh_st* st, int form) {
   if (!st) return fail(CULSH_E_ARG, "null SearchTables");
-  if (form != CULSH_FORM_PLAIN && form != CULSH_FORM_PRECISION)
+  if (form != CULSH_FORM_PLAIN && form != CULSH_FORM_PRECISION &&
+      form != CULSH_FORM_PRECISION_GROUPED)
     return fail(CULSH_E_ARG, "unknown form %d", form);
   st->form = form;
   return CULSH_OK;
@@ -1479,7 +1513,7 @@ int culsh_st_search(culsh_st* st, const float* queries, int Q, const float* data
   st->times.unique = 0.f;
   st->times.distances = (float)(t_cand * f1 / 1000.0);
   st->times.knn = (float)(t_cand * f2 / 1000.0);
-  if (st->form == CULSH_FORM_PRECISION) {
+  if (st->form == CULSH_FORM_PRECISION_GROUPED) {
     // The grouped kernel is not phase-stamped: its whole time is reported
     // as distance time.
     st->times.collect = st->times.knn = 0.f;
@@ -1648,8 +1682,9 @@ extern "C" int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queri
   if (!ht->indexed || !ht->dX) return fail(CULSH_E_STATE, "the index holds no resident dataset");
   if (!d_queries || !d_knn_ids || !d_knn_dists || !d_status)
     return fail(CULSH_E_ARG, "null device pointer");
-  if (st->form != CULSH_FORM_PLAIN)
-    return fail(CULSH_E_ARG, "the allocation-free search runs the plain form only");
+  if (st->form == CULSH_FORM_PRECISION_GROUPED)
+    return fail(CULSH_E_ARG,
+                "the allocation-free search runs direct distances only (not the grouped form)");
   const int L = ht->L, T = st->T, D = ht->D, K = st->K;
   if (T > 1024) return fail(CULSH_E_LIMIT, "T = %d exceeds the supported maximum of 1024", T);
   if ((long long)L * Q * T >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");

@@ -240,21 +240,26 @@ __device__ inline void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
   lo = f2bf_bits(x - bf_bits2f(hi));
 }
 
+constexpr int kBfPad = 40;  // LDS row of 32 bf16 plus 16 bytes of padding
+
 // Precision form: BF16x3 split-precision projection on MFMA
 // (v_mfma_f32_16x16x32_bf16), x.a ~ xh.ah + xh.al + xl.ah with FP32
-// accumulation.  Same tiling and epilogue as k_project_f32.  Error bound:
-// each term is within 3.01 * 2^-16 |x_d a_d| of x_d a_d (dropped xl.al and
-// the split residuals) and the 3D-term FP32 accumulation adds
-// 3.1 D 2^-24 sum |x_d a_d|; with sum |x_d a_d| <= ||x|| ||a_j|| and a
-// factor 2 margin.
+// accumulation.  Same tiling and epilogue as k_project_f32.  A arrives
+// split once at upload (AhT / AlT: [Jp][Dp] bf16, d contiguous per column,
+// Dp = D rounded up to 32, zero padded); each X tile element is split once
+// while it is staged in LDS; every MFMA operand (8 consecutive k of one row
+// or column) is one 16-byte LDS read.  Error bound: each term is within
+// 3.01 * 2^-16 |x_d a_d| of x_d a_d (dropped xl.al and the split residuals)
+// and the 3D-term FP32 accumulation adds 3.1 D 2^-24 sum |x_d a_d|; with
+// sum |x_d a_d| <= ||x|| ||a_j|| and a factor 2 margin.
 __global__ __launch_bounds__(256) void k_project_bf16x3(
-    const float* __restrict__ X, long long N, int D, const float* __restrict__ Ad, int Jp, int J,
-    int M, const float* __restrict__ bj, const float* __restrict__ naj,
-    const float* __restrict__ nx, float W, int16_t* __restrict__ codes,
-    unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
-    unsigned long long seg, int* __restrict__ err) {
-  __shared__ float Xs[kPjRows][kPjKc + 1];
-  __shared__ float As[kPjKc][kPjCols + 1];
+    const float* __restrict__ X, long long N, int D, const uint16_t* __restrict__ AhT,
+    const uint16_t* __restrict__ AlT, int Dp, int Jp, int J, int M,
+    const float* __restrict__ bj, const float* __restrict__ naj, const float* __restrict__ nx,
+    float W, int16_t* __restrict__ codes, unsigned long long* __restrict__ list,
+    unsigned long long* __restrict__ count, unsigned long long seg, int* __restrict__ err) {
+  __shared__ __align__(16) uint16_t Xh[kPjRows][kBfPad], Xl[kPjRows][kBfPad];
+  __shared__ __align__(16) uint16_t Ah[kPjCols][kBfPad], Al[kPjCols][kBfPad];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nct = Jp / kPjCols;
   const long long nrt = (N + kPjRows - 1) / kPjRows;
@@ -266,28 +271,45 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
   floatx4 acc[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // Staging roles: row / column sr, eight consecutive k from sk.
+  const int sr = tid >> 2, sk = (tid & 3) * 8;
+  const bool vecx = (D & 3) == 0;
+  const long long xn = n0 + sr;
   const int kb = (lane >> 4) * 8;  // this lane's 8 consecutive k of the 32
   for (int d0 = 0; d0 < D; d0 += kPjKc) {
-    stage_tile(X, N, D, Ad, Jp, n0, j0, d0, Xs, As);
-    __syncthreads();
-    bf16x8 xh, xl;
+    float xv[8];
+    const int dd = d0 + sk;
+    if (xn < N && vecx && dd + 8 <= D) {
+      const float4* xp = reinterpret_cast<const float4*>(X + xn * D + dd);
+      const float4 a = xp[0], b = xp[1];
+      xv[0] = a.x; xv[1] = a.y; xv[2] = a.z; xv[3] = a.w;
+      xv[4] = b.x; xv[5] = b.y; xv[6] = b.z; xv[7] = b.w;
+    } else {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      uint16_t h, l;
-      split_bf16(Xs[w * 16 + (lane & 15)][kb + i], h, l);
-      xh[i] = __builtin_bit_cast(__bf16, h);
-      xl[i] = __builtin_bit_cast(__bf16, l);
+      for (int i = 0; i < 8; ++i) xv[i] = (xn < N && dd + i < D) ? X[xn * D + dd + i] : 0.f;
     }
+    const uint4 ah4 = *reinterpret_cast<const uint4*>(AhT + (long long)(j0 + sr) * Dp + dd);
+    const uint4 al4 = *reinterpret_cast<const uint4*>(AlT + (long long)(j0 + sr) * Dp + dd);
+    uint32_t hw[4], lw[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      uint16_t h0, l0, h1, l1;
+      split_bf16(xv[2 * i], h0, l0);
+      split_bf16(xv[2 * i + 1], h1, l1);
+      hw[i] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+      lw[i] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+    }
+    *reinterpret_cast<uint4*>(&Xh[sr][sk]) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    *reinterpret_cast<uint4*>(&Xl[sr][sk]) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    *reinterpret_cast<uint4*>(&Ah[sr][sk]) = ah4;
+    *reinterpret_cast<uint4*>(&Al[sr][sk]) = al4;
+    __syncthreads();
+    const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&Xh[w * 16 + (lane & 15)][kb]);
+    const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&Xl[w * 16 + (lane & 15)][kb]);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      bf16x8 ah, al;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        uint16_t h, l;
-        split_bf16(As[kb + i][c * 16 + (lane & 15)], h, l);
-        ah[i] = __builtin_bit_cast(__bf16, h);
-        al[i] = __builtin_bit_cast(__bf16, l);
-      }
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&Ah[c * 16 + (lane & 15)][kb]);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&Al[c * 16 + (lane & 15)][kb]);
       acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, ah, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, al, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, ah, acc[c], 0, 0, 0);
@@ -453,61 +475,65 @@ __global__ __launch_bounds__(256) void k_keys(const int16_t* __restrict__ codes,
   vals[i] = (uint32_t)n;
 }
 
-// Head flags over the sorted (key, id) array.  Equal keys with different
-// codes (a 64-bit hash collision) set *collision; the host then rebuilds
-// with another hash seed.
-__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys,
-                                               const uint32_t* __restrict__ ids, long long n_all,
-                                               long long N, int M, int tb,
-                                               const int16_t* __restrict__ codes,
-                                               uint32_t* __restrict__ flag,
-                                               int* __restrict__ collision) {
+// Head flags over the sorted (key, id) array: a new bucket starts where the
+// key changes.
+__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, long long n_all,
+                                               uint32_t* __restrict__ flag) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n_all) return;
-  const bool head = (i == 0) || keys[i] != keys[i - 1];
-  if (!head) {
-    const int t = (int)(keys[i] >> (64 - tb));
-    const int16_t* a = codes + ((long long)t * N + ids[i]) * M;
-    const int16_t* b = codes + ((long long)t * N + ids[i - 1]) * M;
-    for (int m = 0; m < M; ++m)
-      if (a[m] != b[m]) {
-        atomicOr(collision, 1);
-        break;
-      }
-  }
-  flag[i] = head ? 1u : 0u;
+  flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-// Copies one M-code row, 8 bytes at a time when both rows allow it.
-__device__ inline void copy_codes(int16_t* __restrict__ dst, const int16_t* __restrict__ src,
-                                  int M) {
-  if ((M & 3) == 0) {
-    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
-    uint64_t* d = reinterpret_cast<uint64_t*>(dst);
-    for (int i = 0; i < (M >> 2); ++i) d[i] = s[i];
-  } else if ((M & 1) == 0) {
-    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
-    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-    for (int i = 0; i < (M >> 1); ++i) d[i] = s[i];
-  } else {
-    for (int m = 0; m < M; ++m) dst[m] = src[m];
-  }
-}
-
-// Bucket records from heads: key and start offset into the sorted ids, and
-// codes (row-major [B][M]).
+// Bucket records from heads: key and start offset into the sorted ids (the
+// size follows in k_rec_sizes), and codes (row-major [B][M], M <= 64).  The
+// heads of a block map to consecutive buckets, so records and code rows are
+// gathered into LDS and written out as contiguous runs.
 __global__ __launch_bounds__(256) void k_bucket_fill(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl, long long n_all,
     long long N, int M, int tb, const int16_t* __restrict__ codes, BucketRec* __restrict__ recs,
     int16_t* __restrict__ bcodes) {
+  __shared__ uint4 s_rec[256];
+  __shared__ int16_t s_code[256 * 64];
+  const int tid = threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * 256, i = i0 + tid;
+  const bool head = i < n_all && flag[i] != 0u;
+  const uint32_t kb = excl[i0];
+  const int nh = __syncthreads_count(head);
+  if (head) {
+    const int loc = (int)(excl[i] - kb);
+    const uint64_t key = keys[i];
+    s_rec[loc] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)i, 0u);
+    const int t = (int)(key >> (64 - tb));
+    const int16_t* src = codes + ((long long)t * N + ids[i]) * M;
+    for (int m = 0; m < M; ++m) s_code[loc * M + m] = src[m];
+  }
+  __syncthreads();
+  uint4* rdst = reinterpret_cast<uint4*>(recs + kb);
+  for (int j = tid; j < nh; j += 256) rdst[j] = s_rec[j];
+  int16_t* cdst = bcodes + (long long)kb * M;
+  for (int e = tid; e < nh * M; e += 256) cdst[e] = s_code[e];
+}
+
+// Exactness of the hashed bucket keys: every member's codes must equal its
+// bucket's (two different code vectors with one 64-bit key set *collision;
+// the host then rebuilds with another hash seed).  Members of one bucket are
+// consecutive, so the bucket's row is read once from cache.
+__global__ __launch_bounds__(256) void k_verify_members(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
+    const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl, long long n_all,
+    long long N, int M, int tb, const int16_t* __restrict__ codes,
+    const int16_t* __restrict__ bcodes, int* __restrict__ collision) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_all || !flag[i]) return;
-  const uint32_t k = excl[i];
-  recs[k].key = keys[i];
-  recs[k].start = (uint32_t)i;
+  if (i >= n_all || flag[i]) return;
   const int t = (int)(keys[i] >> (64 - tb));
-  copy_codes(bcodes + (long long)k * M, codes + ((long long)t * N + ids[i]) * M, M);
+  const int16_t* a = codes + ((long long)t * N + ids[i]) * M;
+  const int16_t* b = bcodes + (long long)(excl[i] - 1) * M;
+  for (int m = 0; m < M; ++m)
+    if (a[m] != b[m]) {
+      atomicOr(collision, 1);
+      break;
+    }
 }
 
 // Bucket sizes from consecutive starts.
@@ -531,35 +557,32 @@ __global__ void k_table_bases(const uint32_t* __restrict__ flag, const uint32_t*
   }
 }
 
-__device__ inline long long lower_bound_rec(const BucketRec* __restrict__ recs, long long B,
-                                            uint64_t target) {
-  long long lo = 0, hi = B;
-  while (lo < hi) {
-    const long long mid = (lo + hi) >> 1;
-    if (recs[mid].key < target)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
+// Key directory (see dir_entry) in one pass over the key-sorted records:
+// the first record of each occupied slot writes the slot's entry (first
+// record, fingerprint, count up to 255), and the last record of a slot whose
+// next slot is empty writes that slot's start (so a saturated slot's end is
+// its successor's first record).  Every other slot keeps the empty entry of
+// the fill; slot nslots is the end sentinel (first = B).
+__global__ __launch_bounds__(256) void k_dir_clear(uint64_t* __restrict__ dir, long long n) {
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v < n) dir[v] = 0ull;
 }
 
-// Directory entry of every slot v in [0, nslots] (see dir_entry); slot
-// nslots is the end sentinel (first = B, count 0).
 __global__ __launch_bounds__(256) void k_directory(const BucketRec* __restrict__ recs, long long B,
                                                    int shift, long long nslots,
                                                    uint64_t* __restrict__ dir) {
-  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (v > nslots) return;
-  if (v == nslots) {
-    dir[v] = dir_entry((uint32_t)B, 0, 0);
-    return;
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r == 0) dir[nslots] = dir_entry((uint32_t)B, 0, 0);
+  if (r >= B) return;
+  const uint64_t key = recs[r].key;
+  const long long v = (long long)(key >> shift);
+  if (r == 0 || (long long)(recs[r - 1].key >> shift) != v) {
+    uint32_t cnt = 1;
+    while (cnt < 255 && r + cnt < B && (long long)(recs[r + cnt].key >> shift) == v) ++cnt;
+    dir[v] = dir_entry((uint32_t)r, key, cnt);
   }
-  const long long first = lower_bound_rec(recs, B, (uint64_t)v << shift);
-  const long long end =
-      v + 1 == nslots ? B : lower_bound_rec(recs, B, (uint64_t)(v + 1) << shift);
-  const uint32_t cnt = (uint32_t)(end - first);
-  dir[v] = dir_entry((uint32_t)first, cnt ? recs[first].key : 0ull, cnt);
+  const long long vn = r + 1 < B ? (long long)(recs[r + 1].key >> shift) : nslots;
+  if (vn != v && vn != v + 1 && v + 1 < nslots) dir[v + 1] = dir_entry((uint32_t)(r + 1), 0, 0);
 }
 
 }  // namespace culsh

@@ -5,7 +5,8 @@
 //   k_rs_hist     per-tile digit histogram -> hist[digit][tile]
 //   scan          exclusive scan of hist in digit-major order -> offsets
 //   k_rs_scatter  per-tile stable ranking (wave ballot multi-split, Merrill &
-//                 Garland style) and scatter to offset + rank
+//                 Garland style); the tile is put in digit order in LDS and
+//                 each digit's run is written out contiguously
 // Stability: inside a tile, element order is (wave, item, lane); a wave's
 // running per-digit counters rank its items in that order, waves are
 // prefixed in index order and tiles by the scanned histogram.
@@ -131,10 +132,14 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t* __restrict__
                                                     int shift, const uint32_t* __restrict__ offs,
                                                     long long ntiles) {
   __shared__ uint32_t whist[4][256];
+  __shared__ uint32_t s_lstart[256], s_gstart[256], s_sm4[4];
+  __shared__ uint64_t s_k[kRsTile];
+  __shared__ uint32_t s_v[kRsTile];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 1024; i += 256) (&whist[0][0])[i] = 0;
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kRsTile + (long long)w * 64 * kRsItems;
+  const long long tile0 = (long long)blockIdx.x * kRsTile;
+  const long long base = tile0 + (long long)w * 64 * kRsItems;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   uint64_t k[kRsItems];
   uint32_t v[kRsItems], rank[kRsItems];
@@ -145,6 +150,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t* __restrict__
     k[it] = valid ? kin[idx] : 0ull;
     v[it] = valid ? vin[idx] : 0u;
   }
+  // Rank of each item among the wave's items of its digit, in input order.
 #pragma unroll
   for (int it = 0; it < kRsItems; ++it) {
     const long long idx = base + it * 64 + lane;
@@ -165,26 +171,42 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t* __restrict__
     wave_lds_sync();
   }
   __syncthreads();
+  // Per digit: offsets of the waves' runs, the digit's start in the tile and
+  // its start in the output.
   {
     const int d = threadIdx.x;
-    uint32_t run = offs[(long long)d * ntiles + blockIdx.x];
+    uint32_t run = 0;
 #pragma unroll
     for (int ww = 0; ww < 4; ++ww) {
-      uint32_t c = whist[ww][d];
+      const uint32_t c = whist[ww][d];
       whist[ww][d] = run;
       run += c;
     }
+    uint32_t total;
+    s_lstart[d] = block_exclusive_scan_256(run, s_sm4, total);
+    s_gstart[d] = offs[(long long)d * ntiles + blockIdx.x];
   }
   __syncthreads();
+  // The tile in digit order in LDS (stable), then written out as one
+  // contiguous run per digit.
 #pragma unroll
   for (int it = 0; it < kRsItems; ++it) {
     const long long idx = base + it * 64 + lane;
     if (idx < n) {
       const uint32_t dg = (uint32_t)(k[it] >> shift) & 255u;
-      const uint32_t pos = whist[w][dg] + rank[it];
-      kout[pos] = k[it];
-      vout[pos] = v[it];
+      const uint32_t loc = s_lstart[dg] + whist[w][dg] + rank[it];
+      s_k[loc] = k[it];
+      s_v[loc] = v[it];
     }
+  }
+  __syncthreads();
+  const int cnt = (int)(n - tile0 < kRsTile ? n - tile0 : kRsTile);
+  for (int j = threadIdx.x; j < cnt; j += 256) {
+    const uint64_t kk = s_k[j];
+    const uint32_t dg = (uint32_t)(kk >> shift) & 255u;
+    const uint32_t pos = s_gstart[dg] + (uint32_t)j - s_lstart[dg];
+    kout[pos] = kk;
+    vout[pos] = s_v[j];
   }
 }
 

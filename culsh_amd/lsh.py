@@ -13,8 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (FORM_PLAIN, FORM_PRECISION, Request, Schedule, SrvConfig,  # noqa: F401
-                   SrvRequest, SrvStats, Times)
+from ._lib import (FORM_PLAIN, FORM_PRECISION, FORM_PRECISION_GROUPED,  # noqa: F401
+                   Request, Schedule, SrvConfig, SrvRequest, SrvStats, Times)
 
 
 def _f32(a):

@@ -52,10 +52,18 @@ enum {
   CULSH_E_LIMIT = 7    /* a size limit of this implementation (M > 64, ...) */
 };
 
-/* Search forms (see culsh_st_set_form). */
+/* Forms (culsh_ht_set_form for the index projection, culsh_st_set_form for
+ * the search).  Every form returns the same ids and distances (the FP64
+ * oracle's): FP64 rechecks decide every value near a bucket edge or the
+ * K-th distance. */
 enum {
-  CULSH_FORM_PLAIN = 0,     /* FP32 projection (MFMA), direct FP32 distances */
-  CULSH_FORM_PRECISION = 1  /* BF16x3 projection, norm-expansion distances */
+  CULSH_FORM_PLAIN = 0,      /* FP32 projection (MFMA), direct FP32 distances */
+  CULSH_FORM_PRECISION = 1,  /* BF16x3 split-precision projection (MFMA); direct
+                                FP32 distances */
+  CULSH_FORM_PRECISION_GROUPED = 2  /* search only: distances by the norm
+                                expansion |q|^2 + |x|^2 - 2 q.x on MFMA for groups
+                                of 16 locality-ordered queries (pays only when
+                                the queries of a group share most candidates) */
 };
 
 const char* culsh_last_error(void);
@@ -155,7 +163,8 @@ int culsh_st_wait(culsh_st* st);
  * calls that overlap on the device race (order them on one stream). */
 int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
                                    float* d_knn_dists, int* d_status, void* stream);
-/* Distance form of search(): CULSH_FORM_PLAIN (default) or _PRECISION. */
+/* Distance form of search(): CULSH_FORM_PLAIN (default), _PRECISION (same
+ * direct distances) or _PRECISION_GROUPED (grouped norm expansion). */
 int culsh_st_set_form(culsh_st* st, int form);
 
 /* Results and metrics (cuLSH_SearchTables.cu:83-105). */

@@ -8,8 +8,11 @@ import oracle
 from culsh_amd import HashTables, SearchTables
 
 
-def run_gpu(X, Y, A, b, L, M, W, K, T, form=0, want_buckets=True):
+def run_gpu(X, Y, A, b, L, M, W, K, T, form=0, want_buckets=True, st_form=None):
+    """form: the index projection form (0 plain, 1 precision); st_form: the
+    search form (defaults to form; 2 = grouped norm-expansion distances)."""
     N, D = X.shape
+    st_form = form if st_form is None else st_form
     ht = HashTables()
     assert ht.reset(N, D, L, M, W, A, b), ht.last_error
     if form:
@@ -17,8 +20,8 @@ def run_gpu(X, Y, A, b, L, M, W, K, T, form=0, want_buckets=True):
     assert ht.index(X), ht.last_error
     st = SearchTables()
     assert st.reset(ht, K, T), st.last_error
-    if form:
-        assert st.set_form(form), st.last_error
+    if st_form:
+        assert st.set_form(st_form), st.last_error
     assert st.search(Y, Y.shape[0], X), st.last_error
     cnu, cu = st.getQueryCandidates()
     out = dict(ids=st.getKnnIds(), dists=st.getKnnDistances(), cnu=cnu, cu=cu, ht=ht, st=st)

@@ -205,13 +205,14 @@ def test_config3_slice():
     assert_search_equal(g, o)
 
 
+@pytest.mark.parametrize("st_form", [1, 2])
 @pytest.mark.parametrize("case", ["small_T8", "small_T40_K33", "ties", "c1", "c3_slice",
                                   "large"])
-def test_precision_form_matches_oracle(case):
-    """Precision form: BF16x3 MFMA projection for the index, grouped
-    norm-expansion MFMA distances for the search; FP64 rechecks keep every
-    result bit-exact."""
-    print("precision", case, flush=True)
+def test_precision_form_matches_oracle(case, st_form):
+    """Precision form: BF16x3 MFMA projection for the index; the search with
+    direct distances (1) or grouped norm-expansion MFMA distances (2); FP64
+    rechecks keep every result bit-exact."""
+    print("precision", case, "search form", st_form, flush=True)
     if case in ("small_T8", "small_T40_K33"):
         X, Y, A, b = _small(91)
         L, M, W = 4, 6, 1.0
@@ -235,7 +236,7 @@ def test_precision_form_matches_oracle(case):
         Y = wl.queries_near(X, 40, 0.05, 82)
         A, b = wl.projection(3, 16, 3, 8.0, 83)
         L, M, W, K, T = 3, 3, 8.0, 1500, 2
-    g = run_gpu(X, Y, A, b, L, M, W, K, T, form=1)
+    g = run_gpu(X, Y, A, b, L, M, W, K, T, form=1, st_form=st_form)
     o = run_oracle(X, Y, A, b, L, M, W, K, T)
     assert_buckets_equal(g["ht"], o["index"], L)
     assert_search_equal(g, o)
@@ -338,7 +339,7 @@ def test_random_reset_statistics():
 
 
 def test_precision_search_after_reindex_uses_the_new_dataset():
-    """One SearchTables in the precision form across re-indexing: index(X1),
+    """One SearchTables in the grouped precision form across re-indexing: index(X1),
     then index(X2) of the same shape (its device copy usually lands at the
     same address), then index_device on one tensor whose contents change in
     place.  Every search must use the current dataset's row norms."""
@@ -355,7 +356,7 @@ def test_precision_search_after_reindex_uses_the_new_dataset():
     assert ht.set_form(1)
     st = SearchTables()
     assert st.reset(ht, K, T)
-    assert st.set_form(1)
+    assert st.set_form(2)
     for X in (X1, X2, X1):
         assert ht.index(X), ht.last_error
         assert st.search(Y, Y.shape[0], X), st.last_error

@@ -96,9 +96,9 @@ def test_noalloc_limits():
     d = torch.empty((Q, 10), dtype=torch.float32, device="cuda")
     assert not st.search_device_noalloc(dY, ids, d, status)
     assert "fixed scratch" in st.last_error
-    assert st.set_form(1)
+    assert st.set_form(2)
     assert not st.search_device_noalloc(dY[:10], ids[:10], d[:10], status)
-    assert "plain form" in st.last_error
+    assert "grouped" in st.last_error
     # Every point in one bucket per table: 4 x 300,000 candidates > 2^17.
     rng = np.random.default_rng(7)
     Xb = rng.standard_normal((300000, 4)).astype(np.float32)
