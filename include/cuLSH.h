@@ -86,12 +86,13 @@ class HashTables {
 class SearchTables {
  public:
   typedef struct {
-    float buckets;    // finding the queries' matching buckets
-    float collect;    // gathering candidates from matched buckets
-    float sort;       // ordering candidates
-    float unique;     // removing duplicate candidates
-    float distances;  // distances (on this device: the fused candidate kernel)
-    float knn;        // top-K selection
+    float buckets;    // finding the queries' matching buckets (probe kernel)
+    float collect;    // gathering candidates from matched buckets, deduplicated
+                      // in the same pass (the candidate stage's measured share)
+    float sort;       // locality ordering of large query sets
+    float unique;     // 0: deduplication happens while collecting
+    float distances;  // FP32 distances (share of the candidate stage)
+    float knn;        // K-th selection, FP64 recheck and top-K (share)
     void reset(void) { buckets = collect = sort = unique = distances = knn = 0.0f; }
   } timesStruct;
 
