@@ -140,17 +140,14 @@ BatchArgs batch_args(culsh_bt* bt) {
   a.Ad = ht->dAd.p;
   a.b64 = ht->db64.p;
   a.seeds = ht->dseeds.p;
-  a.dir = ht->dir.p;
+  a.pix = probe_index(ht);
   a.recs = ht->recs.p;
-  a.bcodes = ht->bcodes.p;
   a.sids = ht->sids.p;
   a.W = (double)ht->W;
   a.D = ht->D;
   a.L = ht->L;
   a.M = ht->M;
   a.Jp = ht->Jp;
-  a.tb = ht->tb;
-  a.shift = ht->shift;
   return a;
 }
 

@@ -86,12 +86,11 @@ struct BatchArgs {
   const float* Ad;
   const double* b64;
   const uint64_t* seeds;
-  const uint64_t* dir;
+  ProbeIndex pix;
   const BucketRec* recs;
-  const int16_t* bcodes;
   const uint32_t* sids;
   double W;
-  int D, L, M, Jp, tb, shift;
+  int D, L, M, Jp;
 };
 
 // Expands the request list into per-query descriptors.
@@ -154,8 +153,8 @@ __device__ __forceinline__ void batch_probe(const BatchArgs& a, const BatchHdr& 
       const int t = tt + ti;
       const bool active = qi < ns && t < L;
       const int te = active ? t : 0;
-      probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb, a.shift,
-                         a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * h.Ptot + poff,
+      probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.pix,
+                         orb, a.qb + (long long)te * h.Ptot + poff,
                          a.cnu_qt + (long long)i * L + te);
     }
   }
@@ -276,8 +275,8 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
     const int t = g * tn + ti;
     const bool active = qi < ns && t < L;
     const int te = active ? t : 0;
-    probe8_pair<J, MI>(y, active, te, D, M, C.T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.tb,
-                       a.shift, a.dir, a.recs, a.bcodes, orb, a.qb + (long long)te * Ptot + poff,
+    probe8_pair<J, MI>(y, active, te, D, M, C.T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.pix,
+                       orb, a.qb + (long long)te * Ptot + poff,
                        a.cnu_qt + (long long)i * L + te, te == 0 ? a.lkey + i : nullptr);
   }
 }

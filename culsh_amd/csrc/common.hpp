@@ -178,24 +178,67 @@ __device__ inline uint64_t octet_sum_u64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// ---- Bucket records and the key directory ----
+// ---- Bucket records and the probe index ----
 // One record per bucket: its key, the start of its members in the id array
-// and its size; a lookup hit then needs a single 16-byte load.
+// and its size.  Records are sorted by key (table by table).
 struct __align__(16) BucketRec {
   uint64_t key;
   uint32_t start;
   uint32_t size;
 };
 
-// Directory slot v covers keys with key >> shift == v.  Packed u64:
-//   bits  0-31  index of the slot's first record
-//   bits 32-55  low 24 bits of that record's key (fingerprint)
-//   bits 56-63  number of records in the slot, saturated at 255
+// Probe index: the structures a multiprobe lookup reads.  Most probes ask
+// for a bucket that does not exist, so an absent key must be rejected from
+// cache, and a present one resolved with as few HBM lines as possible.
+//  * Key filter: a register-blocked Bloom filter per table, one 64-bit word
+//    per key (word mulhi(key << tb, nw_t) of the table's nw_t =
+//    ceil(B_t * kFilterBits / 64) words, kFilterHashes bits chosen by a
+//    second hash of the key).  At kFilterBits bits per bucket it holds the
+//    table being probed in the L2 and passes about 1% of absent keys.
+//  * Key directory: slot v covers keys with key >> shift == v, 2^P slots per
+//    table with 2^P >= 2 * (largest table's bucket count), so most slots
+//    hold at most one key.  Packed u64 entry:
+//      bits  0-31  index of the slot's first record
+//      bits 32-55  low 24 bits of that record's key (fingerprint)
+//      bits 56-63  number of records in the slot, saturated at 255
+//  * Bucket lines: one line of lstride (a multiple of 64) bytes per record,
+//    u32 start, u32 size, i16 codes[M]: a directory hit is confirmed by
+//    comparing the probe's full code with the line's and resolved from the
+//    same line.
+constexpr int kFilterBits = 10;
+constexpr int kFilterHashes = 5;
 constexpr uint64_t kFpMask = 0xFFFFFFull;
+
+struct ProbeIndex {
+  const uint64_t* filter;      // key filter words of all tables
+  const uint2* tfw;            // [L]: first filter word, word count of table t
+  const uint64_t* dir;         // [nslots + 1] key directory
+  const unsigned char* lines;  // [B][lstride]: bucket lines in record order
+  const BucketRec* recs;       // key-sorted records (multi-key slots)
+  int tb;                      // table bits of a key
+  int shift;                   // directory slot = key >> shift
+  int lstride;                 // bytes per bucket line
+};
+
+__host__ __device__ inline int bucket_line_bytes(int M) { return (8 + 2 * M + 63) / 64 * 64; }
 
 __device__ __host__ inline uint64_t dir_entry(uint32_t first, uint64_t first_key, uint32_t count) {
   const uint64_t c = count > 255u ? 255u : count;
   return (uint64_t)first | ((first_key & kFpMask) << 32) | (c << 56);
+}
+
+// Filter word of `key` in its table (tfw = the table's first word and count).
+__device__ inline uint32_t filter_word(uint64_t key, int tb, uint2 tfw) {
+  return tfw.x + (uint32_t)__umul64hi(key << tb, (uint64_t)tfw.y);
+}
+
+// The kFilterHashes bits of `key` inside its filter word.
+__device__ inline uint64_t filter_bits(uint64_t key) {
+  const uint64_t h = key * 0x9E3779B97F4A7C15ull;
+  uint64_t m = 0;
+#pragma unroll
+  for (int i = 0; i < kFilterHashes; ++i) m |= 1ull << ((h >> (58 - 6 * i)) & 63);
+  return m;
 }
 
 // Record index of `key`, or -1, given its directory entry e = dir[key >> shift].
@@ -224,9 +267,27 @@ __device__ inline long long lookup_resolve(uint64_t key, uint64_t e,
   return (lo < end && recs[lo].key == key) ? (long long)lo : -1;
 }
 
-__device__ inline long long lookup_key(uint64_t key, const uint64_t* __restrict__ dir, int shift,
-                                       const BucketRec* __restrict__ recs) {
-  return lookup_resolve(key, dir[key >> shift], dir, shift, recs);
+// As lookup_resolve, for a caller that then verifies the bucket's full code:
+// a single-entry slot whose fingerprint matches needs no key load (the
+// slot's keys are all of this table, and equal codes of one table have equal
+// keys).
+__device__ inline long long lookup_unverified(uint64_t key, uint64_t e,
+                                              const uint64_t* __restrict__ dir, int shift,
+                                              const BucketRec* __restrict__ recs) {
+  const uint32_t cnt = (uint32_t)(e >> 56);
+  if (cnt == 0) return -1;
+  if (cnt == 1) return ((e >> 32) & kFpMask) == (key & kFpMask) ? (long long)(uint32_t)e : -1;
+  return lookup_resolve(key, e, dir, shift, recs);
+}
+
+__device__ inline const unsigned char* bucket_line(const ProbeIndex& P, uint32_t g) {
+  return P.lines + (size_t)g * (size_t)P.lstride;
+}
+__device__ inline const int16_t* line_codes(const unsigned char* line) {
+  return reinterpret_cast<const int16_t*>(line + 8);
+}
+__device__ inline uint32_t line_size(const unsigned char* line) {
+  return reinterpret_cast<const uint32_t*>(line)[1];
 }
 
 }  // namespace culsh

@@ -2,7 +2,8 @@
 //
 // Host orchestration for cuLSH::HashTables (cuLSH_HashTables.cu:51-520) and
 // cuLSH::SearchTables (cuLSH_SearchTables.cu:42-463).  The index lives in
-// HBM: dataset, bucket keys, offsets, member ids, codes and a key directory;
+// HBM: dataset, bucket keys, offsets, member ids, codes and the probe index
+// (key filter, key directory and bucket lines);
 // both index() and search() run as a short chain of device kernels on one
 // stream per object.
 #include <hip/hip_runtime.h>
@@ -153,7 +154,11 @@ struct culsh_ht {
   DevBuf<BucketRec> recs;             // [B_total] sorted by key: key, start, size
   DevBuf<uint32_t> sids;              // [L*N] members, bucket by bucket
   DevBuf<int16_t> bcodes;             // [B_total][M]
+  DevBuf<uint64_t> filter;            // key filter words of all tables (ProbeIndex)
+  DevBuf<uint2> tfw;                  // [L] first filter word, word count per table
   DevBuf<uint64_t> dir;               // [nslots + 1] packed directory entries
+  DevBuf<unsigned char> lines;        // [B_total][lstride] bucket lines
+  int lstride = 64;
   std::vector<uint32_t> table_base;   // [L + 1] first global bucket of each table
   int tb = 1, P = 1, shift = 0;
   long long nslots = 0;
@@ -183,7 +188,10 @@ struct culsh_ht {
     recs.release();
     sids.release();
     bcodes.release();
+    filter.release();
+    tfw.release();
     dir.release();
+    lines.release();
     renumber.release();
     table_base.clear();
     hB.clear();
@@ -300,27 +308,63 @@ int set_params(culsh_ht* ht, int N, int D, int L, int M, float W) {
   return CULSH_OK;
 }
 
-// Key directory over the sorted bucket records: 2^P slots per table with
-// 2^P >= 2 * (largest table's bucket count), so most slots hold at most one
-// key and an absent key is usually rejected by one 8-byte load.
-int build_directory(culsh_ht* ht) {
-  long long maxB = 1;
-  for (int t = 0; t < ht->L; ++t)
-    maxB = std::max<long long>(maxB, ht->table_base[t + 1] - ht->table_base[t]);
+// Probe index over the key-sorted bucket records (see ProbeIndex): the key
+// filter (ceil(B_t * kFilterBits / 64) words per table), the key directory
+// (2^P slots per table with 2^P >= 2 * (largest table's bucket count)) and
+// the bucket lines.
+int build_probe_index(culsh_ht* ht) {
+  hipStream_t s = ht->stream;
+  const int L = ht->L, M = ht->M;
+  const long long B = ht->B_total();
+  std::vector<uint2> tfw(L);
+  long long nw = 0, maxB = 1;
+  for (int t = 0; t < L; ++t) {
+    const long long Bt = (long long)ht->table_base[t + 1] - ht->table_base[t];
+    const long long w = std::max<long long>(1, (Bt * kFilterBits + 63) / 64);
+    tfw[t] = make_uint2((uint32_t)nw, (uint32_t)w);
+    nw += w;
+    maxB = std::max(maxB, Bt);
+  }
   ht->P = std::max(1, ceil_log2_host(maxB) + 1);
   ht->P = std::min(ht->P, 30 - ht->tb);  // at most 2^30 slots (8 GiB)
   ht->shift = 64 - ht->tb - ht->P;
   ht->nslots = 1ll << (ht->tb + ht->P);
+  ht->lstride = bucket_line_bytes(M);
+  TRY(ht->tfw.exact(L));
+  TRY(ht->filter.exact((size_t)nw));
   TRY(ht->dir.exact(ht->nslots + 1));
-  hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, ht->stream,
-                     ht->dir.p, ht->nslots + 1);
-  hipLaunchKernelGGL(k_directory, dim3(grid_for(std::max<long long>(ht->B_total(), 1))),
-                     dim3(256), 0, ht->stream, (const BucketRec*)ht->recs.p, ht->B_total(),
-                     ht->shift, ht->nslots, ht->dir.p);
+  TRY(ht->lines.exact((size_t)std::max<long long>(B, 1) * ht->lstride));
+  HIP_TRY(hipMemcpyAsync(ht->tfw.p, tfw.data(), L * sizeof(uint2), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(ht->filter.p, 0, (size_t)nw * 8, s));
+  hipLaunchKernelGGL(k_dir_clear, dim3(grid_for(ht->nslots + 1)), dim3(256), 0, s, ht->dir.p,
+                     ht->nslots + 1);
+  hipLaunchKernelGGL(k_directory, dim3(grid_for(std::max<long long>(B, 1))), dim3(256), 0, s,
+                     (const BucketRec*)ht->recs.p, B, ht->shift, ht->nslots, ht->dir.p);
+  if (B > 0) {
+    hipLaunchKernelGGL(k_filter, dim3(grid_for(B)), dim3(256), 0, s, (const BucketRec*)ht->recs.p,
+                       B, ht->tb, (const uint2*)ht->tfw.p, ht->filter.p);
+    hipLaunchKernelGGL(k_bucket_lines, dim3(grid_for(B)), dim3(256), 0, s,
+                       (const BucketRec*)ht->recs.p, (const int16_t*)ht->bcodes.p, B, M,
+                       ht->lstride, (uint4*)ht->lines.p);
+  }
+  HIP_TRY(hipStreamSynchronize(s));  // tfw's host staging goes out of scope
   return CULSH_OK;
 }
 
-// Keys -> sort -> buckets -> directory, from codes [L][N][M] on the device.
+ProbeIndex probe_index(const culsh_ht* ht) {
+  ProbeIndex P;
+  P.filter = ht->filter.p;
+  P.tfw = ht->tfw.p;
+  P.dir = ht->dir.p;
+  P.lines = ht->lines.p;
+  P.recs = ht->recs.p;
+  P.tb = ht->tb;
+  P.shift = ht->shift;
+  P.lstride = ht->lstride;
+  return P;
+}
+
+// Keys -> sort -> buckets -> probe index, from codes [L][N][M] on the device.
 int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
   hipStream_t s = ht->stream;
   const long long N = ht->N, L = ht->L, M = ht->M, n_all = L * N;
@@ -387,7 +431,7 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
     if (!collision) break;
     if (dbg) fprintf(dbg, "[HashTables::index]\tbucket key collision, reseeding\n");
   }
-  TRY(build_directory(ht));
+  TRY(build_probe_index(ht));
   // The sorted member ids become the index's id array.
   ht->sids.release();
   ht->sids.p = v0.p;
@@ -882,7 +926,7 @@ int index_from_buckets(culsh_ht* ht, FILE* dbg) {
   HIP_TRY(hipMemcpy(ht->bcodes.p, bcodes.data(), bcodes.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ht->sids.p, sids.data(), sids.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ht->renumber.p, renum.data(), Btot * 4, hipMemcpyHostToDevice));
-  TRY(build_directory(ht));
+  TRY(build_probe_index(ht));
   HIP_TRY(hipStreamSynchronize(ht->stream));
   HIP_TRY(hipGetLastError());
   ht->indexed = true;
@@ -1198,6 +1242,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
   culsh_ht* ht = st->ht;
   const int L = ht->L, M = ht->M, D = ht->D, T = st->T, K = st->K;
   if (timed) HIP_TRY(hipEventRecord(st->ev[0], s));
+  const ProbeIndex pix = probe_index(ht);
   const long long pairs = (long long)Q * L;
   const int lds8 = probe8_lds_bytes(D, M);
   if (T <= 128 && lds8 <= 64 * 1024) {
@@ -1206,9 +1251,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 #define CULSH_PROBE8(JJ, MM)                                                                  \
   hipLaunchKernelGGL((k_query_probe8<JJ, MM>), g8, dim3(256), lds8, s, dQ, Q, D, L, M, T,     \
                      (const float*)ht->dAd.p, ht->Jp, (const double*)ht->db64.p, (double)ht->W, \
-                     (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
-                     (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
-                     (const int16_t*)ht->bcodes.p, b.qb, b.cnu_qt)
+                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.cnu_qt)
 #define CULSH_PROBE8_M(JJ)   \
   if (M <= 8)                \
     CULSH_PROBE8(JJ, 1);     \
@@ -1239,9 +1282,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 #define CULSH_PROBE(JJ)                                                                       \
   hipLaunchKernelGGL(k_query_probe<JJ>, pgrid, dim3(256), 0, s, dQ, Q, D, L, M, T,           \
                      (const double*)ht->dAt64.p, (const double*)ht->db64.p, (double)ht->W,   \
-                     (const uint64_t*)ht->dseeds.p, ht->tb, ht->shift,                       \
-                     (const uint64_t*)ht->dir.p, (const BucketRec*)ht->recs.p,               \
-                     (const int16_t*)ht->bcodes.p, b.qb, b.cnu_qt)
+                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.cnu_qt)
     if (T <= 64)
       CULSH_PROBE(1);
     else if (T <= 128)

@@ -1,5 +1,6 @@
 // Index-side kernels: projection -> floor -> code, FP64 boundary recheck,
-// bucket keys, bucket heads, bucket arrays, key directory.
+// bucket keys, bucket heads, bucket arrays, probe index (key filter, key
+// directory, bucket lines).
 //
 // Restates cuLSH_Indexing.cu:181-274 (indexTableData) for all L tables at
 // once: the reference's per-table SGEMM + subtract + divide + floor
@@ -52,28 +53,34 @@ __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X
 }
 
 // Epilogue of a projection tile element: v = (s - b) / W, code = floor(v),
-// and whether the element needs FP64 re-evaluation.  `err_bound` bounds
-// |s - x.a| (the dot product error of the fast path).  Any rounding of the
-// epilogue itself is added below; the decision is taken in FP64.  Returns
-// the recheck decision; invalid elements (outside the tile) return false.
+// and whether the element needs FP64 re-evaluation, all in FP32.  `errb`
+// bounds |s - x.a| (the dot product error of the fast path, already rounded
+// up); rW = fl(1 / W).  With t = (x.a - b) / W exact (b is a float):
+//   |v - t| <= (errb + 2.01 u (|s| + |b|)) rW (1 + 1.01 u) + 1.01 u |v|
+// (u = 2^-24; the subtraction, the rounding of rW and of the product).  eps
+// below exceeds that bound with room for its own six roundings; fr = v -
+// floor(v) is exact except for v in (-1, 0) (error <= 2^-25), hence the
+// extra 2^-24 in the comparison.  An element not flagged has no integer
+// within |v - t| of v, so floor(v) = floor(t); and the FP64 oracle's own
+// error (about D 2^-53 ||x|| ||a_j||) is far below errb, so its floor is the
+// same.  Returns the recheck decision; invalid elements return false.
 __device__ inline bool project_epilogue(bool valid, float s, long long n, int j, long long N,
-                                        int M, float bb, float W, double err_bound,
+                                        int M, float bb, float rW, float errb,
                                         int16_t* __restrict__ codes, int* __restrict__ err) {
   if (!valid) return false;
-  const float v = (s - bb) / W;
-  const double dv = (double)v;
-  const double f = floor(dv);
-  const double u = (double)kU32;
-  const double eps = (err_bound + 2.0 * u * (fabs((double)s) + fabs((double)bb))) / (double)W +
-                     4.0 * u * fabs(dv) + 1e-300;
-  const double fr = dv - f;
-  const int t = j / M, m = j - t * M;
-  if (f < -(double)(kCodeMax - 1) || f > (double)(kCodeMax - 1)) {
+  const float v = (s - bb) * rW;
+  const float f = floorf(v);
+  const float eps =
+      ((errb + kU32 * 3.0f * (fabsf(s) + fabsf(bb))) * rW + kU32 * 2.0f * fabsf(v)) *
+          (1.0f + 0x1.0p-18f) + 1e-30f + kU32;
+  if (f < -(float)(kCodeMax - 1) || f > (float)(kCodeMax - 1)) {
     atomicOr(err, 1);
     return false;
   }
+  const int t = j / M, m = j - t * M;
   codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
-  return fr <= eps || 1.0 - fr <= eps;
+  const float fr = v - f;
+  return fr <= eps || 1.0f - fr <= eps;
 }
 
 constexpr int kRcShards = 8;  // recheck list shards (one counter each)
@@ -93,19 +100,28 @@ __device__ inline void projection_tile_epilogue(
   __shared__ unsigned s_wcnt[4];
   __shared__ unsigned long long s_base;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const float rW = 1.0f / W;
+  // kerr rounded up to FP32; the two FP32 products of errb stay within
+  // (1 + 2^-20) of their exact value.
+  const float kf = (float)kerr * (1.0f + 0x1.0p-20f);
+  float xn[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
+    xn[i] = n < N ? kf * nx[n] : 0.f;
+  }
   unsigned bits = 0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int j = j0 + c * 16 + (lane & 15);
     const bool jv = j < J;
     const float bb = jv ? bj[j] : 0.f;
-    const double na = jv ? (double)naj[j] : 0.0;
+    const float na = jv ? naj[j] * (1.0f + 0x1.0p-20f) : 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
       const bool valid = jv && n < N;
-      if (project_epilogue(valid, acc[c][i], n, j, N, M, bb, W,
-                           valid ? kerr * (double)nx[n] * na : 0.0, codes, err))
+      if (project_epilogue(valid, acc[c][i], n, j, N, M, bb, rW, xn[i] * na, codes, err))
         bits |= 1u << (c * 4 + i);
     }
   }
@@ -154,15 +170,13 @@ __device__ inline bool xcd_tile(int nct, long long ntiles, long long& rt, int& c
 
 constexpr int kPjRows = 64, kPjCols = 64, kPjKc = 32;
 
-// Stages the 64 x 32 X tile and the 32 x 64 A tile of one projection step
-// in LDS.  Every load is issued from a clamped, always-valid address before
-// any is consumed (no per-element branches), then out-of-range elements are
-// zeroed.
-__device__ inline void stage_tile(const float* __restrict__ X, long long N, int D,
-                                  const float* __restrict__ Ad, int Jp, long long n0, int j0,
-                                  int d0, float (*Xs)[kPjKc + 1], float (*As)[kPjCols + 1]) {
+// Loads the 64 x 32 X tile and the 32 x 64 A tile of one projection step
+// into registers: every load is issued from a clamped, always-valid address
+// before any is consumed (no per-element branches).
+__device__ inline void load_tile(const float* __restrict__ X, long long N, int D,
+                                 const float* __restrict__ Ad, int Jp, long long n0, int j0,
+                                 int d0, float (&xv)[8], float (&av)[8]) {
   const int tid = threadIdx.x;
-  float xv[8], av[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int e = tid + 256 * k;
@@ -173,6 +187,13 @@ __device__ inline void stage_tile(const float* __restrict__ X, long long N, int 
     const int ra = e >> 6, ca = e & 63;
     av[k] = Ad[(long long)min(d0 + ra, D - 1) * Jp + j0 + ca];
   }
+}
+
+// Stores a step's registers into LDS, zeroing out-of-range elements.
+__device__ inline void store_tile(long long N, int D, long long n0, int d0, const float (&xv)[8],
+                                  const float (&av)[8], float (*Xs)[kPjKc + 1],
+                                  float (*As)[kPjCols + 1]) {
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int e = tid + 256 * k;
@@ -184,17 +205,19 @@ __device__ inline void stage_tile(const float* __restrict__ X, long long N, int 
 }
 
 // FP32 projection on MFMA (v_mfma_f32_16x16x4_f32): a 64-point x 64-column
-// tile per workgroup, 16 rows per wave, D staged through LDS in chunks of 32.
-// Ad is [Dp][Jp] (column j = t*M + m of table t), zero padded.
-// Plain-form error bound: |s - x.a| <= 2 (D+2) u ||x|| ||a_j||.
+// tile per workgroup, 16 rows per wave, D in steps of 32 through two LDS
+// buffers: the next step's global loads are in flight while the MFMAs of
+// the current one run, and each step takes one barrier.  Ad is [Dp][Jp]
+// (column j = t*M + m of table t), zero padded.  Plain-form error bound:
+// |s - x.a| <= 2 (D+2) u ||x|| ||a_j||.
 __global__ __launch_bounds__(256) void k_project_f32(
     const float* __restrict__ X, long long N, int D, const float* __restrict__ Ad, int Jp, int J,
     int M, const float* __restrict__ bj, const float* __restrict__ naj,
     const float* __restrict__ nx, float W, int16_t* __restrict__ codes,
     unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
     unsigned long long seg, int* __restrict__ err) {
-  __shared__ float Xs[kPjRows][kPjKc + 1];
-  __shared__ float As[kPjKc][kPjCols + 1];
+  __shared__ float Xs[2][kPjRows][kPjKc + 1];
+  __shared__ float As[2][kPjKc][kPjCols + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nct = Jp / kPjCols;
   const long long nrt = (N + kPjRows - 1) / kPjRows;
@@ -206,18 +229,25 @@ __global__ __launch_bounds__(256) void k_project_f32(
   floatx4 acc[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int d0 = 0; d0 < D; d0 += kPjKc) {
-    stage_tile(X, N, D, Ad, Jp, n0, j0, d0, Xs, As);
-    __syncthreads();
+  float xv[8], av[8];
+  load_tile(X, N, D, Ad, Jp, n0, j0, 0, xv, av);
+  store_tile(N, D, n0, 0, xv, av, Xs[0], As[0]);
+  __syncthreads();
+  const int nsteps = (D + kPjKc - 1) / kPjKc;
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < nsteps;
+    if (more) load_tile(X, N, D, Ad, Jp, n0, j0, (st + 1) * kPjKc, xv, av);
 #pragma unroll
     for (int kk = 0; kk < kPjKc; kk += 4) {
-      const float a = Xs[w * 16 + (lane & 15)][kk + (lane >> 4)];
+      const float a = Xs[cur][w * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float b = As[kk + (lane >> 4)][c * 16 + (lane & 15)];
+        const float b = As[cur][kk + (lane >> 4)][c * 16 + (lane & 15)];
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[c], 0, 0, 0);
       }
     }
+    if (more) store_tile(N, D, n0, (st + 1) * kPjKc, xv, av, Xs[cur ^ 1], As[cur ^ 1]);
     __syncthreads();
   }
   const double ka = 2.0 * (double)(D + 2) * (double)kU32;
@@ -557,7 +587,31 @@ __global__ void k_table_bases(const uint32_t* __restrict__ flag, const uint32_t*
   }
 }
 
-// Key directory (see dir_entry) in one pass over the key-sorted records:
+// Probe index (see ProbeIndex), built from the key-sorted records.
+//
+// Key filter: the records of one filter word are consecutive (the word is
+// monotone in the key), so the first record of each run ORs the run's bits
+// and stores the word; the words start zeroed.
+__device__ inline uint32_t rec_filter_word(const BucketRec* __restrict__ recs, long long g, int tb,
+                                           const uint2* __restrict__ tfw) {
+  const uint64_t key = recs[g].key;
+  return filter_word(key, tb, tfw[(int)(key >> (64 - tb))]);
+}
+
+__global__ __launch_bounds__(256) void k_filter(const BucketRec* __restrict__ recs, long long B,
+                                                int tb, const uint2* __restrict__ tfw,
+                                                uint64_t* __restrict__ filter) {
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= B) return;
+  const uint32_t w = rec_filter_word(recs, g, tb, tfw);
+  if (g > 0 && rec_filter_word(recs, g - 1, tb, tfw) == w) return;
+  uint64_t bits = 0;
+  for (long long r = g; r < B && rec_filter_word(recs, r, tb, tfw) == w; ++r)
+    bits |= filter_bits(recs[r].key);
+  filter[w] = bits;
+}
+
+// Key directory (see ProbeIndex) in one pass over the key-sorted records:
 // the first record of each occupied slot writes the slot's entry (first
 // record, fingerprint, count up to 255), and the last record of a slot whose
 // next slot is empty writes that slot's start (so a saturated slot's end is
@@ -583,6 +637,38 @@ __global__ __launch_bounds__(256) void k_directory(const BucketRec* __restrict__
   }
   const long long vn = r + 1 < B ? (long long)(recs[r + 1].key >> shift) : nslots;
   if (vn != v && vn != v + 1 && v + 1 < nslots) dir[v + 1] = dir_entry((uint32_t)(r + 1), 0, 0);
+}
+
+// Bucket lines: one thread per record assembles its line (start, size, the
+// codes in pairs, zero padding) and writes it as 16-byte stores.
+__global__ __launch_bounds__(256) void k_bucket_lines(const BucketRec* __restrict__ recs,
+                                                      const int16_t* __restrict__ bcodes,
+                                                      long long B, int M, int lstride,
+                                                      uint4* __restrict__ lines) {
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= B) return;
+  const BucketRec r = recs[g];
+  const int16_t* c = bcodes + g * M;
+  uint4* out = lines + g * (lstride / 16);
+  for (int ch = 0; ch < lstride / 16; ++ch) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int wi = ch * 4 + i;
+      uint32_t v = 0;
+      if (wi == 0) {
+        v = r.start;
+      } else if (wi == 1) {
+        v = r.size;
+      } else {
+        const int m = 2 * (wi - 2);
+        if (m < M) v = (uint32_t)(uint16_t)c[m];
+        if (m + 1 < M) v |= (uint32_t)(uint16_t)c[m + 1] << 16;
+      }
+      w[i] = v;
+    }
+    out[ch] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 }  // namespace culsh

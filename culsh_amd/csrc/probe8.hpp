@@ -3,9 +3,10 @@
 //
 // Same results as k_query_probe (one wave per pair) -- FP64 projection in the
 // oracle's order, deltas, delta order, the reference's multiprobe queue,
-// probe keys, directory lookup and full-code verify (cuLSH_Querying.cu:63-225,
-// cuLSH_Kernels.cu:124-289) -- but eight pairs share a wave and the queue's
-// cross-lane steps are DPP moves inside the octet.
+// probe keys, filtered directory lookup and full-code verify
+// (cuLSH_Querying.cu:63-225, cuLSH_Kernels.cu:124-289) -- but eight pairs
+// share a wave and the queue's cross-lane steps are DPP moves inside the
+// octet.
 #pragma once
 #include "common.hpp"
 #include "delta_order.hpp"
@@ -31,20 +32,6 @@ __host__ __device__ inline int probe8_mi(int M) {
 __host__ __device__ inline int probe8_lds_bytes(int D, int M) {
   (void)D;
   return 32 * probe8_octet_bytes(M);
-}
-
-// Bucket of probe key `key` with directory entry e, for a caller that then
-// verifies the bucket's full code: a single-entry slot whose fingerprint
-// matches needs no key load (the slot's keys are all of this table, and
-// equal codes of one table have equal keys); larger slots are searched by
-// key.  -1 when absent.
-__device__ inline long long lookup_unverified(uint64_t key, uint64_t e,
-                                              const uint64_t* __restrict__ dir, int shift,
-                                              const BucketRec* __restrict__ recs) {
-  const uint32_t cnt = (uint32_t)(e >> 56);
-  if (cnt == 0) return -1;
-  if (cnt == 1) return ((e >> 32) & kFpMask) == (key & kFpMask) ? (long long)(uint32_t)e : -1;
-  return lookup_resolve(key, e, dir, shift, recs);
 }
 
 // Does the probe code of `mask` (base code plus the mask's +-1 moves) equal
@@ -110,10 +97,8 @@ template <int J, int MI>
 __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, int D, int M, int T,
                                    const float* __restrict__ acol0, int astride,
                                    const double* __restrict__ b64, double W,
-                                   const uint64_t* __restrict__ seeds, int tb, int shift,
-                                   const uint64_t* __restrict__ dir,
-                                   const BucketRec* __restrict__ recs,
-                                   const int16_t* __restrict__ bcodes, unsigned char* orb,
+                                   const uint64_t* __restrict__ seeds, const ProbeIndex& pix,
+                                   unsigned char* orb,
                                    int* __restrict__ qbp, unsigned long long* __restrict__ cnu,
                                    unsigned* __restrict__ cell_key = nullptr) {
   const int lane8 = threadIdx.x & 7;
@@ -316,30 +301,40 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     }
   }
 
-  // Keys, lookup, verify: lane l handles probes l + 8r.  All of a lane's
-  // directory loads are issued before any is resolved.
+  // Keys, lookup, verify: lane l handles probes l + 8r.  Each probe's key
+  // filter word (in cache) is loaded first; only the keys the filter passes
+  // load their directory entry, and a directory hit its bucket line.  All of
+  // a lane's loads of one kind are issued before any is used.
+  const uint2 tfw = pix.tfw[t];
   unsigned long long sizesum = 0;
   uint64_t key[J], ent[J];
 #pragma unroll
   for (int r = 0; r < J; ++r) {
     const int p = r * 8 + lane8;
     key[r] = (active && p < T)
-                 ? make_key(probe_hash(basesum, seed, base, perm, rank, M, pm[r]), t, tb)
+                 ? make_key(probe_hash(basesum, seed, base, perm, rank, M, pm[r]), t, pix.tb)
                  : 0ull;
   }
 #pragma unroll
   for (int r = 0; r < J; ++r) {
     const int p = r * 8 + lane8;
-    ent[r] = (active && p < T) ? dir[key[r] >> shift] : 0ull;
+    ent[r] = (active && p < T) ? pix.filter[filter_word(key[r], pix.tb, tfw)] : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < J; ++r) {
+    const int p = r * 8 + lane8;
+    const uint64_t fb = filter_bits(key[r]);
+    ent[r] = (active && p < T && (ent[r] & fb) == fb) ? pix.dir[key[r] >> pix.shift] : 0ull;
   }
 #pragma unroll
   for (int r = 0; r < J; ++r) {
     const int p = r * 8 + lane8;
     if (active && p < T) {
-      long long k = lookup_unverified(key[r], ent[r], dir, shift, recs);
+      long long k = lookup_unverified(key[r], ent[r], pix.dir, pix.shift, pix.recs);
       if (k >= 0) {
-        const uint32_t size = recs[k].size;  // in flight with the code loads
-        if (probe_matches(bcodes + k * M, base, perm, rank, M, pm[r]))
+        const unsigned char* line = bucket_line(pix, (uint32_t)k);
+        const uint32_t size = line_size(line);  // in flight with the code loads
+        if (probe_matches(line_codes(line), base, perm, rank, M, pm[r]))
           sizesum += size;
         else
           k = -1;
@@ -353,16 +348,16 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
 
 // The search's octet kernel: 32 queries of one table per workgroup, tables
 // in sequence (workgroup b: table b / ceil(Q/32)), so the workgroups in
-// flight look up one or two tables' directory, records and codes at a time
+// flight look up one or two tables' filters, directories and bucket lines at a
+// time
 // and those stay in cache.  qb[t][q*T + p], cnu_qt[q*L + t].  Needs
 // probe8_lds_bytes(D, M) bytes of dynamic LDS.
 template <int J, int MI>
 __global__ __launch_bounds__(256) void k_query_probe8(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
     const float* __restrict__ Ad, int Jp, const double* __restrict__ b64, double W,
-    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
-    const BucketRec* __restrict__ recs, const int16_t* __restrict__ bcodes,
-    int* __restrict__ qb, unsigned long long* __restrict__ cnu_qt) {
+    const uint64_t* __restrict__ seeds, ProbeIndex pix, int* __restrict__ qb,
+    unsigned long long* __restrict__ cnu_qt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, oct = tid >> 3;
   const int nqb = (Q + 31) / 32;
@@ -375,7 +370,7 @@ __global__ __launch_bounds__(256) void k_query_probe8(
   // least 64 floats of padding after its last row).
   const int qa = active ? q : q0;
   probe8_pair<J, MI>(Qm + (long long)qa * D, active, t, D, M, T, Ad + t * M, Jp, b64, W,
-                     seeds, tb, shift, dir, recs, bcodes, orb,
+                     seeds, pix, orb,
                      qb + (long long)t * Q * T + (long long)qa * T, cnu_qt + (long long)qa * L + t);
 }
 

@@ -1,8 +1,8 @@
 // Query-side kernels.
 //
 // k_query_probe  one wave per (query, table): FP64 projection, deltas,
-//                delta order, the multiprobe queue, probe keys, directory +
-//                binary-search lookup with a full-code verify.  Restates
+//                delta order, the multiprobe queue, probe keys, filtered
+//                directory lookup with a full-code verify.  Restates
 //                cuLSH_Querying.cu:171-225 (findTableQueryBins),
 //                Querying.cu:63-119 + Kernels.cu:188-289 (multiprobe) and
 //                Querying.cu:124-165 + Kernels.cu:124-138 (lookup).
@@ -29,9 +29,7 @@ template <int J>
 __global__ __launch_bounds__(256) void k_query_probe(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
     const double* __restrict__ At64, const double* __restrict__ b64, double W,
-    const uint64_t* __restrict__ seeds, int tb, int shift, const uint64_t* __restrict__ dir,
-    const BucketRec* __restrict__ recs,
-    const int16_t* __restrict__ bcodes, int* __restrict__ qb,
+    const uint64_t* __restrict__ seeds, ProbeIndex pix, int* __restrict__ qb,
     unsigned long long* __restrict__ cnu_qt) {
   __shared__ double s_ds[4][64];
   __shared__ double s_d[4][128];
@@ -190,6 +188,7 @@ __global__ __launch_bounds__(256) void k_query_probe(
   }
 
   // Probe keys, lookup, verify.
+  const uint2 tfw = pix.tfw[t];
   unsigned long long sizesum = 0;
   const long long qbBase = (long long)t * Q * T + (long long)q * T;
 #pragma unroll
@@ -210,9 +209,16 @@ __global__ __launch_bounds__(256) void k_query_probe(
         const int b0 = sbase[dim];
         hs += code_term(seed, dim, b0 + (di >= M ? 1 : -1)) - code_term(seed, dim, b0);
       }
-      long long k = lookup_key(make_key(hs, t, tb), dir, shift, recs);
+      const uint64_t key = make_key(hs, t, pix.tb);
+      const uint64_t fb = filter_bits(key);
+      long long k = -1;
+      uint32_t size = 0;
+      if ((pix.filter[filter_word(key, pix.tb, tfw)] & fb) == fb)
+        k = lookup_unverified(key, pix.dir[key >> pix.shift], pix.dir, pix.shift, pix.recs);
       if (k >= 0) {
-        const int16_t* bc = bcodes + k * M;
+        const unsigned char* line = bucket_line(pix, (uint32_t)k);
+        const int16_t* bc = line_codes(line);
+        size = line_size(line);
         for (int m = 0; m < M; ++m) {
           const int rd = rank[m], ru = rank[M + m];
           int c = sbase[m];
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(256) void k_query_probe(
         }
       }
       qb[qbBase + p] = (int)k;
-      if (k >= 0) sizesum += recs[k].size;
+      if (k >= 0) sizesum += size;
     }
   }
   sizesum = wave_sum_u64(sizesum);
