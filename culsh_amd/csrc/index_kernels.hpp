@@ -171,36 +171,54 @@ __device__ inline bool xcd_tile(int nct, long long ntiles, long long& rt, int& c
 constexpr int kPjRows = 64, kPjCols = 64, kPjKc = 32;
 
 // Loads the 64 x 32 X tile and the 32 x 64 A tile of one projection step
-// into registers: every load is issued from a clamped, always-valid address
-// before any is consumed (no per-element branches).
+// into registers as 16-byte vectors (thread e of 512 takes row e >> 3 of X,
+// columns 4 (e & 7) .. +3, and row e >> 4 of A, columns 4 (e & 15) .. +3).
+// Every load is issued from a clamped, always-valid address before any is
+// consumed; elements past D or N are zeroed when stored.  X's rows are read
+// as vectors when D % 4 == 0 and X is 16-byte aligned (then a vector either
+// lies inside the row or wholly past it), else element by element.
 __device__ inline void load_tile(const float* __restrict__ X, long long N, int D,
                                  const float* __restrict__ Ad, int Jp, long long n0, int j0,
-                                 int d0, float (&xv)[8], float (&av)[8]) {
+                                 int d0, float4 (&xv)[2], float4 (&av)[2]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 2; ++k) {
     const int e = tid + 256 * k;
-    const int r = e >> 5, c = e & 31;
+    const int r = e >> 3, c = (e & 7) * 4;
     const long long n = min(n0 + r, N - 1);
-    const int d = min(d0 + c, D - 1);
-    xv[k] = X[n * D + d];
-    const int ra = e >> 6, ca = e & 63;
-    av[k] = Ad[(long long)min(d0 + ra, D - 1) * Jp + j0 + ca];
+    if ((D & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
+      const int d = min(d0 + c, D - 4);
+      xv[k] = *reinterpret_cast<const float4*>(X + n * D + d);
+    } else {
+      const float* xr = X + n * D;
+      xv[k] = make_float4(xr[min(d0 + c, D - 1)], xr[min(d0 + c + 1, D - 1)],
+                          xr[min(d0 + c + 2, D - 1)], xr[min(d0 + c + 3, D - 1)]);
+    }
+    const int ra = e >> 4, ca = (e & 15) * 4;
+    av[k] = *reinterpret_cast<const float4*>(Ad + (long long)min(d0 + ra, D - 1) * Jp + j0 + ca);
   }
 }
 
 // Stores a step's registers into LDS, zeroing out-of-range elements.
-__device__ inline void store_tile(long long N, int D, long long n0, int d0, const float (&xv)[8],
-                                  const float (&av)[8], float (*Xs)[kPjKc + 1],
+__device__ inline void store_tile(long long N, int D, long long n0, int d0, const float4 (&xv)[2],
+                                  const float4 (&av)[2], float (*Xs)[kPjKc + 1],
                                   float (*As)[kPjCols + 1]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 2; ++k) {
     const int e = tid + 256 * k;
-    const int r = e >> 5, c = e & 31;
-    Xs[r][c] = (n0 + r < N && d0 + c < D) ? xv[k] : 0.f;
-    const int ra = e >> 6, ca = e & 63;
-    As[ra][ca] = (d0 + ra < D) ? av[k] : 0.f;
+    const int r = e >> 3, c = (e & 7) * 4;
+    const bool rv = n0 + r < N;
+    Xs[r][c] = (rv && d0 + c < D) ? xv[k].x : 0.f;
+    Xs[r][c + 1] = (rv && d0 + c + 1 < D) ? xv[k].y : 0.f;
+    Xs[r][c + 2] = (rv && d0 + c + 2 < D) ? xv[k].z : 0.f;
+    Xs[r][c + 3] = (rv && d0 + c + 3 < D) ? xv[k].w : 0.f;
+    const int ra = e >> 4, ca = (e & 15) * 4;
+    const bool av_ok = d0 + ra < D;
+    As[ra][ca] = av_ok ? av[k].x : 0.f;
+    As[ra][ca + 1] = av_ok ? av[k].y : 0.f;
+    As[ra][ca + 2] = av_ok ? av[k].z : 0.f;
+    As[ra][ca + 3] = av_ok ? av[k].w : 0.f;
   }
 }
 
@@ -229,7 +247,7 @@ __global__ __launch_bounds__(256) void k_project_f32(
   floatx4 acc[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
-  float xv[8], av[8];
+  float4 xv[2], av[2];
   load_tile(X, N, D, Ad, Jp, n0, j0, 0, xv, av);
   store_tile(N, D, n0, 0, xv, av, Xs[0], As[0]);
   __syncthreads();
@@ -639,35 +657,42 @@ __global__ __launch_bounds__(256) void k_directory(const BucketRec* __restrict__
   if (vn != v && vn != v + 1 && v + 1 < nslots) dir[v + 1] = dir_entry((uint32_t)(r + 1), 0, 0);
 }
 
-// Bucket lines: one thread per record assembles its line (start, size, the
-// codes in pairs, zero padding) and writes it as 16-byte stores.
+// Bucket lines (start, size, the codes in pairs, zero padding): a
+// workgroup copies the code rows of its 256 records into LDS with
+// coalesced loads, then writes their lines as consecutive 16-byte chunks.
 __global__ __launch_bounds__(256) void k_bucket_lines(const BucketRec* __restrict__ recs,
                                                       const int16_t* __restrict__ bcodes,
                                                       long long B, int M, int lstride,
                                                       uint4* __restrict__ lines) {
-  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (g >= B) return;
-  const BucketRec r = recs[g];
-  const int16_t* c = bcodes + g * M;
-  uint4* out = lines + g * (lstride / 16);
-  for (int ch = 0; ch < lstride / 16; ++ch) {
+  __shared__ int16_t s_code[256 * 64];
+  const int tid = threadIdx.x;
+  const long long g0 = (long long)blockIdx.x * 256;
+  const int nb = (int)min(256ll, B - g0);
+  const int16_t* src = bcodes + g0 * M;
+  for (int e = tid; e < nb * M; e += 256) s_code[e] = src[e];
+  __syncthreads();
+  const int cpl = lstride / 16;
+  uint4* dst = lines + g0 * cpl;
+  for (int e = tid; e < nb * cpl; e += 256) {
+    const int r = e / cpl, ch = e - r * cpl;
+    const BucketRec rec = recs[g0 + r];
     uint32_t w[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int wi = ch * 4 + i;
       uint32_t v = 0;
       if (wi == 0) {
-        v = r.start;
+        v = rec.start;
       } else if (wi == 1) {
-        v = r.size;
+        v = rec.size;
       } else {
         const int m = 2 * (wi - 2);
-        if (m < M) v = (uint32_t)(uint16_t)c[m];
-        if (m + 1 < M) v |= (uint32_t)(uint16_t)c[m + 1] << 16;
+        if (m < M) v = (uint32_t)(uint16_t)s_code[r * M + m];
+        if (m + 1 < M) v |= (uint32_t)(uint16_t)s_code[r * M + m + 1] << 16;
       }
       w[i] = v;
     }
-    out[ch] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[e] = make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
