@@ -254,49 +254,58 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
       qs[0] = (uint64_t)__double_as_longlong(ds[0]);
     }
     int size = 1;
-    for (int p = 1; p < T; ++p) {
-      // Pop: the lowest slot (s = 8 r + lane) among the minimal scores.
-      uint64_t omin = ~0ull;
-      Code probe = 0;
-      int br = 0;
+    // Pop p = 8 r + j runs with r fixed at compile time, so the popped set
+    // goes to register r of lane j and, while every earlier pop made a new
+    // entry (size == p), so does the new shift entry (slot p).
 #pragma unroll
-      for (int r = 0; r < J; ++r)
-        if (qs[r] < omin) {
-          omin = qs[r];
-          br = r;
-          probe = qc[r];
-        }
-      uint32_t ws = (uint32_t)(br * 8 + lane8);
-      octet_argmin_u64(omin, ws, probe);
-      const int wl = (int)(ws & 7), wr = (int)(ws >> 3);
-      if (lane8 == (p & 7)) {
+    for (int r = 0; r < J; ++r) {
+      for (int j = (r == 0 ? 1 : 0); j < 8; ++j) {
+        const int p = r * 8 + j;
+        if (p >= T) break;  // uniform over the wave
+        // Pop: the lowest slot (s = 8 i + lane) among the minimal scores.
+        uint64_t omin = ~0ull;
+        Code probe = 0;
+        int bi = 0;
 #pragma unroll
-        for (int r = 0; r < J; ++r)
-          if (r == (p >> 3)) pm[r] = probe;
-      }
-      const int maxPos = kTop - (J <= 4 ? __clz((int)probe) : __clzll((long long)probe));
-      if (maxPos < maxT - 1) {
-        const double e = __longlong_as_double((long long)omin) + ds[maxPos + 1];
-        const Code pe = probe | ((Code)1 << (maxPos + 1));
-        const double sh = e - ds[maxPos];
-        const Code ps = pe - ((Code)1 << maxPos);
-        if (lane8 == wl) {
+        for (int i = 0; i < J; ++i)
+          if (qs[i] < omin) {
+            omin = qs[i];
+            bi = i;
+            probe = qc[i];
+          }
+        uint32_t ws = (uint32_t)(bi * 8 + lane8);
+        octet_argmin_u64(omin, ws, probe);
+        const int wl = (int)(ws & 7), wi = (int)(ws >> 3);
+        if (lane8 == j) pm[r] = probe;
+        const int maxPos = kTop - (J <= 4 ? __clz((int)probe) : __clzll((long long)probe));
+        if (maxPos < maxT - 1) {
+          const double e = __longlong_as_double((long long)omin) + ds[maxPos + 1];
+          const Code pe = probe | ((Code)1 << (maxPos + 1));
+          const double sh = e - ds[maxPos];
+          const Code ps = pe - ((Code)1 << maxPos);
+          if (lane8 == wl) {
 #pragma unroll
-          for (int r = 0; r < J; ++r)
-            if (r == wr) {
-              qc[r] = pe;
-              qs[r] = (uint64_t)__double_as_longlong(e);
-            }
-        }
-        if (lane8 == (size & 7)) {
-#pragma unroll
-          for (int r = 0; r < J; ++r)
-            if (r == (size >> 3)) {
+            for (int i = 0; i < J; ++i)
+              if (i == wi) {
+                qc[i] = pe;
+                qs[i] = (uint64_t)__double_as_longlong(e);
+              }
+          }
+          if (size == p) {
+            if (lane8 == j) {
               qc[r] = ps;
               qs[r] = (uint64_t)__double_as_longlong(sh);
             }
+          } else if (lane8 == (size & 7)) {
+#pragma unroll
+            for (int i = 0; i < J; ++i)
+              if (i == (size >> 3)) {
+                qc[i] = ps;
+                qs[i] = (uint64_t)__double_as_longlong(sh);
+              }
+          }
+          ++size;
         }
-        ++size;
       }
     }
   }
