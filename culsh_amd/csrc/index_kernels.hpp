@@ -15,6 +15,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr float kU32 = 5.9604644775390625e-08f;  // 2^-24, FP32 unit roundoff
 constexpr int kCodeMax = 32767;
+constexpr int kPjRows = 64, kPjCols = 64, kPjKc = 32;  // projection tile and d-step
 
 // Upper bound of the Euclidean norm of each row (16 lanes per row).
 __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, long long N, int D,
@@ -63,10 +64,10 @@ __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X
 // extra 2^-24 in the comparison.  An element not flagged has no integer
 // within |v - t| of v, so floor(v) = floor(t); and the FP64 oracle's own
 // error (about D 2^-53 ||x|| ||a_j||) is far below errb, so its floor is the
-// same.  Returns the recheck decision; invalid elements return false.
-__device__ inline bool project_epilogue(bool valid, float s, long long n, int j, long long N,
-                                        int M, float bb, float rW, float errb,
-                                        int16_t* __restrict__ codes, int* __restrict__ err) {
+// same.  Writes the code to *code; returns the recheck decision (invalid
+// elements: false, code untouched).
+__device__ inline bool project_epilogue(bool valid, float s, float bb, float rW, float errb,
+                                        int16_t* code, int* __restrict__ err) {
   if (!valid) return false;
   const float v = (s - bb) * rW;
   const float f = floorf(v);
@@ -77,26 +78,61 @@ __device__ inline bool project_epilogue(bool valid, float s, long long n, int j,
     atomicOr(err, 1);
     return false;
   }
-  const int t = j / M, m = j - t * M;
-  codes[((long long)t * N + n) * M + m] = (int16_t)(int)f;
+  *code = (int16_t)(int)f;
   const float fr = v - f;
   return fr <= eps || 1.0f - fr <= eps;
+}
+
+// Writes a 64 x 64 tile of codes staged in LDS (sc[r * 64 + c] for point
+// n0 + r, column j0 + c) to codes [L][N][M]: table by table, the tile's
+// rows of one table are one run of M-code rows, so consecutive threads
+// write consecutive code pairs (4-byte stores when M and the table's first
+// column in the tile are even, else one code per thread).
+__device__ inline void store_code_tile(const int16_t* sc, long long n0, int j0, long long N,
+                                       int J, int M, int16_t* __restrict__ codes) {
+  const int tid = threadIdx.x;
+  const int rows = (int)min((long long)kPjRows, N - n0);
+  const int jend = min(j0 + kPjCols, J);
+  for (int t = j0 / M; t * M < jend; ++t) {
+    const int ms = max(j0 - t * M, 0), me = min(jend - t * M, M);  // m range in the tile
+    const int cs = t * M + ms - j0;                                // its first tile column
+    int16_t* dst = codes + ((long long)t * N + n0) * M;
+    if ((M & 1) == 0 && (ms & 1) == 0 && (me & 1) == 0) {
+      const int np = (me - ms) >> 1;
+      for (int e = tid; e < rows * np; e += 256) {
+        const int r = e / np, q = e - r * np;
+        const int c = cs + 2 * q;
+        const uint32_t v = (uint32_t)(uint16_t)sc[r * kPjCols + c] |
+                           ((uint32_t)(uint16_t)sc[r * kPjCols + c + 1] << 16);
+        *reinterpret_cast<uint32_t*>(dst + (long long)r * M + ms + 2 * q) = v;
+      }
+    } else {
+      const int nm = me - ms;
+      for (int e = tid; e < rows * nm; e += 256) {
+        const int r = e / nm, q = e - r * nm;
+        dst[(long long)r * M + ms + q] = sc[r * kPjCols + cs + q];
+      }
+    }
+  }
 }
 
 constexpr int kRcShards = 8;  // recheck list shards (one counter each)
 
 // Epilogue of a 64 x 64 projection tile held as MFMA accumulators (16 rows
-// per wave, lane layout of v_mfma_f32_16x16x*): codes, then the flagged
-// (n, j) appended to recheck-list shard blockIdx % kRcShards with one
-// atomic per workgroup.  Shard k holds entries list[k * seg, k * seg +
-// count[k]); a count above seg means the shard overflowed.  `kerr` is the
-// fast path's dot product error factor (bound = kerr ||x|| ||a_j||).  Every
-// thread of the workgroup must call it.
+// per wave, lane layout of v_mfma_f32_16x16x*): codes (staged in the 8 KB
+// of LDS at sc, which the caller no longer needs, then written by
+// store_code_tile), then the flagged (n, j) appended to recheck-list shard
+// blockIdx % kRcShards with one atomic per workgroup.  Shard k holds entries
+// list[k * seg, k * seg + count[k]); a count above seg means the shard
+// overflowed.  `kerr` is the fast path's dot product error factor (bound =
+// kerr ||x|| ||a_j||).  Every thread of the workgroup must call it, after a
+// barrier that ends every read of sc's previous contents.
 __device__ inline void projection_tile_epilogue(
     const floatx4 (&acc)[4], long long n0, int j0, long long N, int J, int M,
     const float* __restrict__ bj, const float* __restrict__ naj, const float* __restrict__ nx,
-    float W, double kerr, int16_t* __restrict__ codes, unsigned long long* __restrict__ list,
-    unsigned long long* __restrict__ count, unsigned long long seg, int* __restrict__ err) {
+    float W, double kerr, int16_t* sc, int16_t* __restrict__ codes,
+    unsigned long long* __restrict__ list, unsigned long long* __restrict__ count,
+    unsigned long long seg, int* __restrict__ err) {
   __shared__ unsigned s_wcnt[4];
   __shared__ unsigned long long s_base;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -119,12 +155,15 @@ __device__ inline void projection_tile_epilogue(
     const float na = jv ? naj[j] * (1.0f + 0x1.0p-20f) : 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const long long n = n0 + w * 16 + (lane >> 4) * 4 + i;
-      const bool valid = jv && n < N;
-      if (project_epilogue(valid, acc[c][i], n, j, N, M, bb, rW, xn[i] * na, codes, err))
+      const int r = w * 16 + (lane >> 4) * 4 + i;
+      const bool valid = jv && n0 + r < N;
+      if (project_epilogue(valid, acc[c][i], bb, rW, xn[i] * na, &sc[r * kPjCols + c * 16 + (lane & 15)],
+                           err))
         bits |= 1u << (c * 4 + i);
     }
   }
+  __syncthreads();
+  store_code_tile(sc, n0, j0, N, J, M, codes);
   // Offsets: inclusive scan of the per-lane counts within the wave, then
   // per-wave totals, then one atomic for the workgroup.
   const unsigned lc = (unsigned)__popc(bits);
@@ -168,7 +207,6 @@ __device__ inline bool xcd_tile(int nct, long long ntiles, long long& rt, int& c
   return true;
 }
 
-constexpr int kPjRows = 64, kPjCols = 64, kPjKc = 32;
 
 // Loads the 64 x 32 X tile and the 32 x 64 A tile of one projection step
 // into registers as 16-byte vectors (thread e of 512 takes row e >> 3 of X,
@@ -269,8 +307,8 @@ __global__ __launch_bounds__(256) void k_project_f32(
     __syncthreads();
   }
   const double ka = 2.0 * (double)(D + 2) * (double)kU32;
-  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, ka, codes, list, count,
-                           seg, err);
+  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, ka,
+                           reinterpret_cast<int16_t*>(&Xs[0][0][0]), codes, list, count, seg, err);
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -306,8 +344,11 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
     const float* __restrict__ bj, const float* __restrict__ naj, const float* __restrict__ nx,
     float W, int16_t* __restrict__ codes, unsigned long long* __restrict__ list,
     unsigned long long* __restrict__ count, unsigned long long seg, int* __restrict__ err) {
-  __shared__ __align__(16) uint16_t Xh[kPjRows][kBfPad], Xl[kPjRows][kBfPad];
-  __shared__ __align__(16) uint16_t Ah[kPjCols][kBfPad], Al[kPjCols][kBfPad];
+  __shared__ __align__(16) uint16_t s_bf[4][kPjRows][kBfPad];  // Xh, Xl, Ah, Al
+  uint16_t(*Xh)[kBfPad] = s_bf[0];
+  uint16_t(*Xl)[kBfPad] = s_bf[1];
+  uint16_t(*Ah)[kBfPad] = s_bf[2];
+  uint16_t(*Al)[kBfPad] = s_bf[3];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nct = Jp / kPjCols;
   const long long nrt = (N + kPjRows - 1) / kPjRows;
@@ -365,8 +406,9 @@ __global__ __launch_bounds__(256) void k_project_bf16x3(
     __syncthreads();
   }
   const double kb3 = 2.0 * (3.01 * 0x1.0p-16 + 3.1 * (double)D * 0x1.0p-24);
-  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, kb3, codes, list, count,
-                           seg, err);
+  projection_tile_epilogue(acc, n0, j0, N, J, M, bj, naj, nx, W, kb3,
+                           reinterpret_cast<int16_t*>(&s_bf[0][0][0]), codes, list, count, seg,
+                           err);
 }
 
 // Re-evaluates the recorded projections in FP64 in the oracle's operation

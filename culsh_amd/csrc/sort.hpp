@@ -212,6 +212,85 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t* __restrict__
 
 inline long long rs_tiles(long long n) { return (n + kRsTile - 1) / kRsTile; }
 
+// Completes a stable sort by key bits [shift, 64) into a stable sort by the
+// whole key.  A run of equal key >> shift needs work only when it holds an
+// inversion (a key above its successor): k_sort_inversions lists the first
+// element of each such run (exactly one thread per run lists it: the one at
+// the run's first inversion), then k_sort_runs insertion-sorts each listed
+// run by the full key, stably, one thread per run.  Runs are disjoint, and
+// permuting a run leaves key >> shift unchanged at every position of it.
+// The list is sharded by workgroup (kSortShards shards of sort_shard_cap(n)
+// entries: a shard's workgroups hold at most that many elements) and
+// appended once per wave.
+constexpr int kSortShards = 64;
+inline long long sort_shard_cap(long long n) {
+  const long long blocks = (n + 255) / 256;
+  return (blocks + kSortShards - 1) / kSortShards * 256;
+}
+
+__global__ __launch_bounds__(256) void k_sort_inversions(const uint64_t* __restrict__ keys,
+                                                         long long n, int shift,
+                                                         uint32_t* __restrict__ list,
+                                                         long long cap,
+                                                         unsigned* __restrict__ count) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  bool mine = false;
+  long long h = 0;
+  if (i > 0 && i < n) {
+    const uint64_t a = keys[i - 1], b = keys[i];
+    const uint64_t hi = b >> shift;
+    if ((a >> shift) == hi && a > b) {
+      mine = true;
+      h = i - 1;
+      while (h > 0 && (keys[h - 1] >> shift) == hi) {
+        if (keys[h - 1] > keys[h]) {  // an earlier inversion owns the run
+          mine = false;
+          break;
+        }
+        --h;
+      }
+    }
+  }
+  const unsigned long long m = __ballot(mine);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+  const int shard = (int)(blockIdx.x % kSortShards);
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(&count[shard], (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  if (mine)
+    list[(long long)shard * cap + base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] =
+        (uint32_t)h;
+}
+
+// Grid: kSortShards x parts workgroups; workgroup b takes shard b % kSortShards.
+__global__ __launch_bounds__(256) void k_sort_runs(uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals, long long n,
+                                                   int shift, const uint32_t* __restrict__ list,
+                                                   long long cap,
+                                                   const unsigned* __restrict__ count) {
+  const int shard = (int)(blockIdx.x % kSortShards);
+  const int part = (int)(blockIdx.x / kSortShards), parts = (int)(gridDim.x / kSortShards);
+  const unsigned nruns = count[shard];
+  for (unsigned r = part * 256 + threadIdx.x; r < nruns; r += parts * 256) {
+    const long long h = list[(long long)shard * cap + r];
+    const uint64_t hi = keys[h] >> shift;
+    for (long long a = h + 1; a < n && (keys[a] >> shift) == hi; ++a) {
+      const uint64_t k = keys[a];
+      if (keys[a - 1] <= k) continue;
+      const uint32_t v = vals[a];
+      long long b = a;
+      while (b > h && keys[b - 1] > k) {
+        keys[b] = keys[b - 1];
+        vals[b] = vals[b - 1];
+        --b;
+      }
+      keys[b] = k;
+      vals[b] = v;
+    }
+  }
+}
+
 // Scratch (u32 elements) for radix_sort_pairs on n elements.
 inline long long rs_scratch_elems(long long n) {
   long long h = 256 * rs_tiles(n);
