@@ -432,16 +432,12 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
     const long long B = ht->table_base[L];
     TRY(ht->recs.exact(B));
     TRY(ht->bcodes.exact(B * M));
+    HIP_TRY(hipMemsetAsync(dflags.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n_all)), dim3(256), 0, s,
                        (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
                        (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p, ht->recs.p,
-                       ht->bcodes.p);
+                       ht->bcodes.p, dflags.p);
     hipLaunchKernelGGL(k_rec_sizes, dim3(grid_for(B)), dim3(256), 0, s, ht->recs.p, B, n_all);
-    HIP_TRY(hipMemsetAsync(dflags.p, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_verify_members, dim3(grid_for(n_all)), dim3(256), 0, s,
-                       (const uint64_t*)k0.p, (const uint32_t*)v0.p, (const uint32_t*)flag.p,
-                       (const uint32_t*)excl.p, n_all, N, (int)M, ht->tb, codes.p,
-                       (const int16_t*)ht->bcodes.p, dflags.p);
     int collision = 0;
     HIP_TRY(hipMemcpyAsync(&collision, dflags.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

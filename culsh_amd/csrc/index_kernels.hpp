@@ -574,56 +574,69 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
   flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-// Bucket records from heads: key and start offset into the sorted ids (the
-// size follows in k_rec_sizes), and codes (row-major [B][M], M <= 64).  The
-// heads of a block map to consecutive buckets, so records and code rows are
-// gathered into LDS and written out as contiguous runs.
+// Bucket records, bucket codes and the exactness check of the hashed keys in
+// one pass over the sorted (key, id) array.  Every element's code row is
+// gathered once into LDS; heads write their bucket's record (key, start;
+// the size follows in k_rec_sizes) and code row (row-major [B][M], M <= 64)
+// as contiguous runs; every other element compares its codes with its
+// predecessor's (same key), so all members of a bucket equal its head by
+// induction -- two different code vectors with one 64-bit key set
+// *collision and the host rebuilds with another hash seed.  The first
+// element of a workgroup also gathers the row before it.
 __global__ __launch_bounds__(256) void k_bucket_fill(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl, long long n_all,
     long long N, int M, int tb, const int16_t* __restrict__ codes, BucketRec* __restrict__ recs,
-    int16_t* __restrict__ bcodes) {
+    int16_t* __restrict__ bcodes, int* __restrict__ collision) {
   __shared__ uint4 s_rec[256];
-  __shared__ int16_t s_code[256 * 64];
+  __shared__ unsigned char s_hloc[256];
+  __shared__ int16_t s_code[257 * 64];  // row r + 1: element i0 + r; row 0: element i0 - 1
   const int tid = threadIdx.x;
   const long long i0 = (long long)blockIdx.x * 256, i = i0 + tid;
-  const bool head = i < n_all && flag[i] != 0u;
+  const bool in = i < n_all;
+  const bool head = in && flag[i] != 0u;
   const uint32_t kb = excl[i0];
   const int nh = __syncthreads_count(head);
+  auto gather = [&](long long e, int row) {
+    const int t = (int)(keys[e] >> (64 - tb));
+    const int16_t* src = codes + ((long long)t * N + ids[e]) * M;
+    int16_t* dst = s_code + row * M;
+    if ((M & 1) == 0) {
+      const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src);
+      for (int m = 0; m < M / 2; ++m) {
+        const uint32_t v = s32[m];
+        dst[2 * m] = (int16_t)(v & 0xFFFFu);
+        dst[2 * m + 1] = (int16_t)(v >> 16);
+      }
+    } else {
+      for (int m = 0; m < M; ++m) dst[m] = src[m];
+    }
+  };
+  if (in) gather(i, tid + 1);
+  if (tid == 0 && in && !head) gather(i - 1, 0);  // i > 0: element 0 is a head
   if (head) {
     const int loc = (int)(excl[i] - kb);
     const uint64_t key = keys[i];
     s_rec[loc] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)i, 0u);
-    const int t = (int)(key >> (64 - tb));
-    const int16_t* src = codes + ((long long)t * N + ids[i]) * M;
-    for (int m = 0; m < M; ++m) s_code[loc * M + m] = src[m];
+    s_hloc[loc] = (unsigned char)tid;
   }
   __syncthreads();
+  if (in && !head) {
+    const int16_t* a = s_code + (tid + 1) * M;
+    const int16_t* p = s_code + tid * M;
+    for (int m = 0; m < M; ++m)
+      if (a[m] != p[m]) {
+        atomicOr(collision, 1);
+        break;
+      }
+  }
   uint4* rdst = reinterpret_cast<uint4*>(recs + kb);
   for (int j = tid; j < nh; j += 256) rdst[j] = s_rec[j];
   int16_t* cdst = bcodes + (long long)kb * M;
-  for (int e = tid; e < nh * M; e += 256) cdst[e] = s_code[e];
-}
-
-// Exactness of the hashed bucket keys: every member's codes must equal its
-// bucket's (two different code vectors with one 64-bit key set *collision;
-// the host then rebuilds with another hash seed).  Members of one bucket are
-// consecutive, so the bucket's row is read once from cache.
-__global__ __launch_bounds__(256) void k_verify_members(
-    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
-    const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl, long long n_all,
-    long long N, int M, int tb, const int16_t* __restrict__ codes,
-    const int16_t* __restrict__ bcodes, int* __restrict__ collision) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_all || flag[i]) return;
-  const int t = (int)(keys[i] >> (64 - tb));
-  const int16_t* a = codes + ((long long)t * N + ids[i]) * M;
-  const int16_t* b = bcodes + (long long)(excl[i] - 1) * M;
-  for (int m = 0; m < M; ++m)
-    if (a[m] != b[m]) {
-      atomicOr(collision, 1);
-      break;
-    }
+  for (int e = tid; e < nh * M; e += 256) {
+    const int j = e / M, m = e - j * M;
+    cdst[e] = s_code[((int)s_hloc[j] + 1) * M + m];
+  }
 }
 
 // Bucket sizes from consecutive starts.
