@@ -18,7 +18,8 @@ struct culsh_bt {
   unsigned char* hpack = nullptr;  // pinned host image of dpack
   BatchHdr* hres = nullptr;        // pinned: header after the batch
   DevBuf<BatchQuery> dbq;
-  DevBuf<int> dqb, dovl, dorder;
+  DevBuf<uint2> dqr;  // probe member ranges
+  DevBuf<int> dovl, dorder;
   DevBuf<unsigned> dlhist, dlkey;
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q;
   DevBuf<uint32_t> dscr;
@@ -128,7 +129,7 @@ BatchArgs batch_args(culsh_bt* bt) {
   a.hdr = reinterpret_cast<BatchHdr*>(bt->dpack.p);
   a.classes = reinterpret_cast<const BatchClass*>(bt->dpack.p + bt->off_cls);
   a.bq = bt->dbq.p;
-  a.qb = bt->dqb.p;
+  a.qr = bt->dqr.p;
   a.cnu_qt = bt->dcnu_qt.p;
   a.cnu_q = bt->dcnu_q.p;
   a.cu_q = bt->dcu_q.p;
@@ -141,7 +142,6 @@ BatchArgs batch_args(culsh_bt* bt) {
   a.b64 = ht->db64.p;
   a.seeds = ht->dseeds.p;
   a.pix = probe_index(ht);
-  a.recs = ht->recs.p;
   a.sids = ht->sids.p;
   a.W = (double)ht->W;
   a.D = ht->D;
@@ -194,7 +194,7 @@ int bt_reserve(culsh_bt* bt, int Qtot, int nreq, long long Ptot) {
   }
   if (grow_p) {
     bt->Pcap = 0;
-    TRY(bt->dqb.exact((size_t)L * Pc));
+    TRY(bt->dqr.exact((size_t)L * Pc));
     bt->Pcap = Pc;
   }
   if (need_h) TRY(bt->dlhist.exact(kLocBins));

@@ -73,7 +73,7 @@ struct BatchArgs {
   BatchHdr* hdr;
   const BatchClass* classes;
   const BatchQuery* bq;
-  int* qb;
+  uint2* qr;  // [L][Ptot] member ranges of the probes (start, size)
   unsigned long long* cnu_qt;
   unsigned long long* cnu_q;
   unsigned long long* cu_q;
@@ -87,7 +87,6 @@ struct BatchArgs {
   const double* b64;
   const uint64_t* seeds;
   ProbeIndex pix;
-  const BucketRec* recs;
   const uint32_t* sids;
   double W;
   int D, L, M, Jp;
@@ -154,7 +153,7 @@ __device__ __forceinline__ void batch_probe(const BatchArgs& a, const BatchHdr& 
       const bool active = qi < ns && t < L;
       const int te = active ? t : 0;
       probe8_pair<J, MI>(y, active, te, D, M, T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.pix,
-                         orb, a.qb + (long long)te * h.Ptot + poff,
+                         orb, nullptr, a.qr + (long long)te * h.Ptot + poff,
                          a.cnu_qt + (long long)i * L + te);
     }
   }
@@ -210,14 +209,14 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
       const BatchQuery d = a.bq[i];
       QueryIO io;
       io.y = h.Qm + d.qrow * a.D;
-      io.qbq = a.qb + d.poff;
+      io.qrq = a.qr + d.poff;
       io.tstride = h.Ptot;
       io.T = d.T;
       io.K = d.K;
       io.oid = h.out_ids + d.out;
       io.od = h.out_d + d.out;
       io.slot = i;
-      if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+      if (!query_candidates<R>(io, a.X, a.D, a.L, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
                                &hp->n_recheck, B)) {
         if (tid == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
       }
@@ -276,7 +275,7 @@ __global__ __launch_bounds__(256) void k_batch_probe(BatchArgs a) {
     const bool active = qi < ns && t < L;
     const int te = active ? t : 0;
     probe8_pair<J, MI>(y, active, te, D, M, C.T, a.Ad + te * M, a.Jp, a.b64, a.W, a.seeds, a.pix,
-                       orb, a.qb + (long long)te * Ptot + poff,
+                       orb, nullptr, a.qr + (long long)te * Ptot + poff,
                        a.cnu_qt + (long long)i * L + te, te == 0 ? a.lkey + i : nullptr);
   }
 }
@@ -403,14 +402,14 @@ __global__ __launch_bounds__(256, 3) void k_batch_cand(BatchArgs a) {
     const BatchQuery d = a.bq[i];
     QueryIO io;
     io.y = Qm + d.qrow * a.D;
-    io.qbq = a.qb + d.poff;
+    io.qrq = a.qr + d.poff;
     io.tstride = Ptot;
     io.T = d.T;
     io.K = d.K;
     io.oid = out_ids + d.out;
     io.od = out_d + d.out;
     io.slot = i;
-    if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+    if (!query_candidates<R>(io, a.X, a.D, a.L, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
                              &hp->n_recheck, B)) {
       if (tid == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
     }
@@ -444,14 +443,14 @@ __global__ __launch_bounds__(256, 3) void k_batch_cand_each(BatchArgs a) {
   const BatchQuery d = a.bq[i];
   QueryIO io;
   io.y = hp->Qm + d.qrow * a.D;
-  io.qbq = a.qb + d.poff;
+  io.qrq = a.qr + d.poff;
   io.tstride = hp->Ptot;
   io.T = d.T;
   io.K = d.K;
   io.oid = hp->out_ids + d.out;
   io.od = hp->out_d + d.out;
   io.slot = i;
-  if (!query_candidates<R>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+  if (!query_candidates<R>(io, a.X, a.D, a.L, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
                            &hp->n_recheck, B)) {
     if (threadIdx.x == 0) a.ovl[atomicAdd(&hp->n_overflow, 1)] = i;
   }
@@ -493,14 +492,14 @@ __global__ __launch_bounds__(256) void k_batch_overflow(BatchArgs a, uint32_t* _
     B.ys = s_y;
     QueryIO io;
     io.y = h.Qm + d.qrow * a.D;
-    io.qbq = a.qb + d.poff;
+    io.qrq = a.qr + d.poff;
     io.tstride = h.Ptot;
     io.T = d.T;
     io.K = d.K;
     io.oid = h.out_ids + d.out;
     io.od = h.out_d + d.out;
     io.slot = i;
-    if (!query_candidates<4>(io, a.X, a.D, a.L, a.recs, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
+    if (!query_candidates<4>(io, a.X, a.D, a.L, a.sids, a.cnu_qt, a.cnu_q, a.cu_q,
                              &hp->n_recheck, B)) {
       if (threadIdx.x == 0) atomicOr(&hp->err, 1);
     }

@@ -177,37 +177,30 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
 
 // Calls ins(id) for every member of every bucket a query matched
 // (cuLSH_SearchTables.cu:321-340).  Wave w takes the (table, probe) entries
-// w*64 + lane + 256 i; the probe slots and bucket records of kE such rounds
-// are loaded together, then per round the wave prefix-sums the bucket sizes
-// and spreads the member ids over its lanes, kR rounds of 64 members with
-// their id loads in flight together.  qbq[t * tstride + p] is the query's
-// probe p of table t.  All threads of the workgroup must call it.
+// w*64 + lane + 256 i; the member ranges of kE such rounds are loaded
+// together, then per round the wave prefix-sums the bucket sizes and spreads
+// the member ids over its lanes, kR rounds of 64 members with their id loads
+// in flight together.  qrq[t * tstride + p] is the member range (start,
+// size; size 0 when nothing matched) of the query's probe p of table t.
+// All threads of the workgroup must call it.
 template <class Ins>
-__device__ inline void for_each_member(const int* __restrict__ qbq, long long tstride, int L,
-                                       int T, const BucketRec* __restrict__ recs,
-                                       const uint32_t* __restrict__ sids, Ins ins) {
+__device__ inline void for_each_member(const uint2* __restrict__ qrq, long long tstride, int L,
+                                       int T, const uint32_t* __restrict__ sids, Ins ins) {
   constexpr int kE = 8, kR = 8;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int LT = L * T;
   for (int p0 = w * 64; p0 < LT; p0 += 256 * kE) {
-    int kb[kE];
-#pragma unroll
-    for (int i = 0; i < kE; ++i) {
-      const int e = p0 + 256 * i + lane;
-      kb[i] = -1;
-      if (e < LT) {
-        const int t = e / T, p = e - t * T;
-        kb[i] = qbq[(long long)t * tstride + p];
-      }
-    }
     uint32_t st[kE], sz[kE];
 #pragma unroll
     for (int i = 0; i < kE; ++i) {
+      const int e = p0 + 256 * i + lane;
       st[i] = 0;
       sz[i] = 0;
-      if (kb[i] >= 0) {
-        st[i] = recs[kb[i]].start;
-        sz[i] = recs[kb[i]].size;
+      if (e < LT) {
+        const int t = e / T, p = e - t * T;
+        const uint2 r = qrq[(long long)t * tstride + p];
+        st[i] = r.x;
+        sz[i] = r.y;
       }
     }
 #pragma unroll
@@ -445,7 +438,7 @@ struct CandBufs {
 // (cnu_qt[slot * L + t]) and its cnu_q / cu_q entries.
 struct QueryIO {
   const float* y;       // the query vector (D floats)
-  const int* qbq;       // its probe 0 of table 0; probe p of table t at t * tstride + p
+  const uint2* qrq;     // member range of its probe 0 of table 0; probe p of table t at t * tstride + p
   long long tstride;
   int T, K;
   int* oid;             // its K result ids and distances
@@ -454,10 +447,10 @@ struct QueryIO {
 };
 
 __device__ inline QueryIO search_io(int q, const float* Qm, int Q, int D, int T, int K,
-                                    const int* qb, int* out_ids, float* out_d) {
+                                    const uint2* qr, int* out_ids, float* out_d) {
   QueryIO io;
   io.y = Qm + (long long)q * D;
-  io.qbq = qb + (long long)q * T;
+  io.qrq = qr + (long long)q * T;
   io.tstride = (long long)Q * T;
   io.T = T;
   io.K = K;
@@ -471,8 +464,7 @@ __device__ inline QueryIO search_io(int q, const float* Qm, int Q, int D, int T,
 // fit the buffers; its outputs are then not written.
 template <int R = 4>
 __device__ inline bool query_candidates(const QueryIO& io, const float* __restrict__ X, int D,
-                                        int L, const BucketRec* __restrict__ recs,
-                                        const uint32_t* __restrict__ sids,
+                                        int L, const uint32_t* __restrict__ sids,
                                         const unsigned long long* __restrict__ cnu_qt,
                                         unsigned long long* __restrict__ cnu_q,
                                         unsigned long long* __restrict__ cu_q,
@@ -514,7 +506,7 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
   __syncthreads();
   if (cnu > 0) {
-    for_each_member(io.qbq, io.tstride, L, io.T, recs, sids, [&](uint32_t id) {
+    for_each_member(io.qrq, io.tstride, L, io.T, sids, [&](uint32_t id) {
       uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
       for (uint32_t probe = 0;; ++probe) {
         if (probe == hslots) {
@@ -575,8 +567,8 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
 // order in sequence, so queries sharing candidate rows run on one L2.
 __global__ __launch_bounds__(256, 3) void k_candidates(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
-    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    const uint2* __restrict__ qr, const uint32_t* __restrict__ sids,
+    const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, int* __restrict__ overflow,
     int* __restrict__ n_overflow, unsigned long long* __restrict__ n_recheck,
@@ -602,8 +594,8 @@ __global__ __launch_bounds__(256, 3) void k_candidates(
     if (pos >= Q) return;
     q = (int)order[pos];
   }
-  if (!query_candidates<4>(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
-                        cnu_qt, cnu_q, cu_q, n_recheck, B)) {
+  if (!query_candidates<4>(search_io(q, Qm, Q, D, T, K, qr, out_ids, out_d), X, D, L, sids,
+                           cnu_qt, cnu_q, cu_q, n_recheck, B)) {
     if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1)] = q;
   }
 }
@@ -623,8 +615,8 @@ __global__ __launch_bounds__(256) void k_max_cnu(const int* __restrict__ list, i
 // *err.
 __global__ __launch_bounds__(256) void k_candidates_global(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
-    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    const uint2* __restrict__ qr, const uint32_t* __restrict__ sids,
+    const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, const int* __restrict__ list, int n_list,
     uint32_t* __restrict__ scratch, int hbits, unsigned ucap,
@@ -646,8 +638,8 @@ __global__ __launch_bounds__(256) void k_candidates_global(
   B.ys = s_y;
   B.phase = phase;
   for (int i = blockIdx.x; i < n_list; i += gridDim.x) {
-    if (!query_candidates(search_io(list[i], Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs,
-                          sids, cnu_qt, cnu_q, cu_q, n_recheck, B)) {
+    if (!query_candidates(search_io(list[i], Qm, Q, D, T, K, qr, out_ids, out_d), X, D, L, sids,
+                          cnu_qt, cnu_q, cu_q, n_recheck, B)) {
       if (threadIdx.x == 0) atomicOr(err, 1);
     }
   }
@@ -666,8 +658,8 @@ constexpr unsigned kOvAsyncUcap = 1u << 20;
 // FP64 rechecks than kScapG, sets *err (sticky: the host clears it).
 __global__ __launch_bounds__(256) void k_candidates_overflow(
     const float* __restrict__ X, const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
-    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    const uint2* __restrict__ qr, const uint32_t* __restrict__ sids,
+    const unsigned long long* __restrict__ cnu_qt,
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, const int* __restrict__ list,
     const int* __restrict__ n_list, uint32_t* __restrict__ scratch, int hbits_max,
@@ -696,7 +688,7 @@ __global__ __launch_bounds__(256) void k_candidates_overflow(
     B.sid = s_sid;
     B.scap = kScapG;
     B.ys = s_y;
-    if (!query_candidates(search_io(q, Qm, Q, D, T, K, qb, out_ids, out_d), X, D, L, recs, sids,
+    if (!query_candidates(search_io(q, Qm, Q, D, T, K, qr, out_ids, out_d), X, D, L, sids,
                           cnu_qt, cnu_q, cu_q, n_recheck, B)) {
       if (threadIdx.x == 0) atomicOr(err, 1);
     }
@@ -766,8 +758,8 @@ __global__ __launch_bounds__(256) void k_group_cnu(const uint32_t* __restrict__ 
 __global__ __launch_bounds__(256, 3) void k_group_candidates(
     const float* __restrict__ X, const float* __restrict__ xn2, float xmax,
     const float* __restrict__ Qm, int Q, int D, int L, int T, int K,
-    const int* __restrict__ qb, const BucketRec* __restrict__ recs,
-    const uint32_t* __restrict__ sids, const unsigned long long* __restrict__ cnu_qt,
+    const uint2* __restrict__ qr, const uint32_t* __restrict__ sids,
+    const unsigned long long* __restrict__ cnu_qt,
     const uint32_t* __restrict__ order, int* __restrict__ out_ids, float* __restrict__ out_d,
     unsigned long long* __restrict__ cnu_q, unsigned long long* __restrict__ cu_q,
     uint32_t* __restrict__ scratch, size_t region, int hbits, unsigned qcap,
@@ -828,7 +820,7 @@ __global__ __launch_bounds__(256, 3) void k_group_candidates(
     // Union of the group's candidates; hash slot masks record membership.
     for (int g = 0; g < gs; ++g) {
       const int q = s_qid[g];
-      for_each_member(qb + (long long)q * T, (long long)Q * T, L, T, recs, sids, [&](uint32_t id) {
+      for_each_member(qr + (long long)q * T, (long long)Q * T, L, T, sids, [&](uint32_t id) {
         uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
         for (uint32_t probe = 0;; ++probe) {
           if (probe == hslots) {

@@ -289,5 +289,9 @@ __device__ inline const int16_t* line_codes(const unsigned char* line) {
 __device__ inline uint32_t line_size(const unsigned char* line) {
   return reinterpret_cast<const uint32_t*>(line)[1];
 }
+// Member range of a bucket line: (start, size).
+__device__ inline uint2 line_range(const unsigned char* line) {
+  return *reinterpret_cast<const uint2*>(line);
+}
 
 }  // namespace culsh

@@ -1013,6 +1013,7 @@ struct culsh_st {
   int max_pass = 0;  // queries per pass cap (0: from the free HBM)
   DevBuf<float> dQ, dX_alt, ddists;
   DevBuf<int> dqb, dids, dov, dqb_local;
+  DevBuf<uint2> dqr;  // probe member ranges (start, size) for the candidate stage
   DevBuf<unsigned long long> dcnu_qt, dcnu_q, dcu_q, dcnt, dmax;
   DevBuf<uint32_t> dscratch;  // working sets of overflowed queries
   DevBuf<int> dnov;
@@ -1132,8 +1133,7 @@ int run_overflow(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_i
   TRY(st->dscratch.ensure(region * G));
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
   hipLaunchKernelGGL(k_candidates_global, dim3(G), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
-                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
-                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                     (const uint2*)st->dqr.p, (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
                      d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p, nov,
                      st->dscratch.p, hbits, ucap, st->dcnt.p, st->dnov.p,
                      st->stamp ? st->dphase.p : nullptr);
@@ -1227,9 +1227,8 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
   }
   hipLaunchKernelGGL(k_group_candidates, dim3(G), dim3(256), 0, s, dX,
                      (const float*)st->dxn2.p, st->xmax, dQ, Q, D, L, T, K,
-                     (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
-                     (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
-                     order, d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, st->dgscratch.p, region,
+                     (const uint2*)st->dqr.p, (const uint32_t*)ht->sids.p,
+                     (const unsigned long long*)st->dcnu_qt.p, order, d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, st->dgscratch.p, region,
                      hbits, qcap, st->dov.p, st->dnov.p, st->dcnt.p);
   HIP_TRY(hipGetLastError());
   return CULSH_OK;
@@ -1237,7 +1236,8 @@ int run_group_candidates(culsh_st* st, const float* dQ, const float* dX, int Q, 
 
 // Working buffers of one device search.
 struct SearchBufs {
-  int* qb;                          // [L][Q*T] probe buckets
+  int* qb;                          // [L][Q*T] probe buckets, or null
+  uint2* qr;                        // [L][Q*T] their member ranges (start, size)
   unsigned long long* cnu_qt;       // [Q*L]
   unsigned long long *cnu_q, *cu_q; // [Q]
   int* ov;                          // [Q] overflowed queries
@@ -1264,7 +1264,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 #define CULSH_PROBE8(JJ, MM)                                                                  \
   hipLaunchKernelGGL((k_query_probe8<JJ, MM>), g8, dim3(256), lds8, s, dQ, Q, D, L, M, T,     \
                      (const float*)ht->dAd.p, ht->Jp, (const double*)ht->db64.p, (double)ht->W, \
-                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.cnu_qt)
+                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.qr, b.cnu_qt)
 #define CULSH_PROBE8_M(JJ)   \
   if (M <= 8)                \
     CULSH_PROBE8(JJ, 1);     \
@@ -1295,7 +1295,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 #define CULSH_PROBE(JJ)                                                                       \
   hipLaunchKernelGGL(k_query_probe<JJ>, pgrid, dim3(256), 0, s, dQ, Q, D, L, M, T,           \
                      (const double*)ht->dAt64.p, (const double*)ht->db64.p, (double)ht->W,   \
-                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.cnu_qt)
+                     (const uint64_t*)ht->dseeds.p, pix, b.qb, b.qr, b.cnu_qt)
     if (T <= 64)
       CULSH_PROBE(1);
     else if (T <= 128)
@@ -1325,8 +1325,7 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
     }
     if (timed) HIP_TRY(hipEventRecord(st->ev[3], s));
     hipLaunchKernelGGL(k_candidates, dim3(grid), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
-                       (const int*)b.qb, (const BucketRec*)ht->recs.p,
-                       (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt, d_ids,
+                       (const uint2*)b.qr, (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt, d_ids,
                        d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order, b.phase);
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
@@ -1353,6 +1352,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   const long long QT = (long long)Q * T;
   if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
   TRY(st->dqb.ensure((size_t)L * QT));
+  TRY(st->dqr.ensure((size_t)L * QT));
   TRY(st->dcnu_qt.ensure((size_t)Q * L));
   TRY(st->dcnu_q.ensure(Q));
   TRY(st->dcu_q.ensure(Q));
@@ -1362,15 +1362,16 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   TRY(st->dmax.ensure(1));
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
-  SearchBufs b{st->dqb.p, st->dcnu_qt.p, st->dcnu_q.p, st->dcu_q.p, st->dov.p, st->dnov.p,
-               st->dcnt.p, true, st->stamp ? st->dphase.p : nullptr};
+  SearchBufs b{st->dqb.p,  st->dqr.p,  st->dcnu_qt.p, st->dcnu_q.p,
+               st->dcu_q.p, st->dov.p, st->dnov.p,   st->dcnt.p,
+               true,       st->stamp ? st->dphase.p : nullptr};
   TRY(launch_probe_and_candidates(st, dQ, dX, Q, d_ids, d_dists, s, b, timed));
   if (async) {
     TRY(st->dovfix.ensure((size_t)kOvAsyncGrid *
                           ((1ull << kOvAsyncHbits) + 2ull * kOvAsyncUcap)));
     hipLaunchKernelGGL(k_candidates_overflow, dim3(kOvAsyncGrid), dim3(256), 0, s, dX, dQ, Q, D,
-                       L, T, K, (const int*)st->dqb.p, (const BucketRec*)ht->recs.p,
-                       (const uint32_t*)ht->sids.p, (const unsigned long long*)st->dcnu_qt.p,
+                       L, T, K, (const uint2*)st->dqr.p, (const uint32_t*)ht->sids.p,
+                       (const unsigned long long*)st->dcnu_qt.p,
                        d_ids, d_dists, st->dcnu_q.p, st->dcu_q.p, (const int*)st->dov.p,
                        (const int*)st->dnov.p, st->dovfix.p, kOvAsyncHbits, kOvAsyncUcap,
                        st->dcnt.p, st->derr.p);
@@ -1445,10 +1446,11 @@ int search_pass(culsh_st* st, int Q, bool host, int* out) {
   culsh_ht* ht = st->ht;
   const long long L = ht->L, T = st->T, K = st->K, D = ht->D;
   long long q = std::min<long long>(Q, ((1ll << 31) - 1) / (L * T));
-  const long long per = 4 * L * T + 8 * L + 16 + 4 + 48 + (host ? 4 * L * T + 4 * D + 8 * K : 0);
+  const long long per = 12 * L * T + 8 * L + 16 + 4 + 48 + (host ? 4 * L * T + 4 * D + 8 * K : 0);
   size_t freeb = 0, totalb = 0;
   HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-  long long held = (long long)(st->dqb.n * 4 + st->dcnu_qt.n * 8 + st->dcnu_q.n * 16);
+  long long held =
+      (long long)(st->dqb.n * 4 + st->dqr.n * 8 + st->dcnu_qt.n * 8 + st->dcnu_q.n * 16);
   if (host) held += (long long)(st->dqb_local.n * 4 + st->dQ.n * 4 + st->dids.n * 4 + st->ddists.n * 4);
   q = std::min<long long>(q, ((long long)(freeb / 2) + held) / per);
   if (st->max_pass > 0) q = std::min<long long>(q, st->max_pass);
@@ -1705,7 +1707,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Byte offsets of the search buffers in the scratch array.
 struct NaLayout {
-  size_t ovfix, ctr, cnu_q, cu_q, ov, cnu_qt, qb, total;
+  size_t ovfix, ctr, cnu_q, cu_q, ov, cnu_qt, qr, total;
 };
 
 NaLayout na_layout(long long Q, int L, int T) {
@@ -1716,8 +1718,8 @@ NaLayout na_layout(long long Q, int L, int T) {
   o.cu_q = align256(o.cnu_q + 8ull * Q);
   o.ov = align256(o.cu_q + 8ull * Q);
   o.cnu_qt = align256(o.ov + 4ull * Q);
-  o.qb = align256(o.cnu_qt + 8ull * Q * L);
-  o.total = o.qb + 4ull * Q * T * L;
+  o.qr = align256(o.cnu_qt + 8ull * Q * L);
+  o.total = o.qr + 8ull * Q * T * L;
   return o;
 }
 
@@ -1759,15 +1761,14 @@ extern "C" int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queri
   int* nov = reinterpret_cast<int*>(base + lay.ctr);
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.ctr + 8);
   hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(64), 0, s, nov, cnt, d_status);
-  SearchBufs b{reinterpret_cast<int*>(base + lay.qb),
+  SearchBufs b{nullptr, reinterpret_cast<uint2*>(base + lay.qr),
                reinterpret_cast<unsigned long long*>(base + lay.cnu_qt),
                reinterpret_cast<unsigned long long*>(base + lay.cnu_q),
                reinterpret_cast<unsigned long long*>(base + lay.cu_q),
                reinterpret_cast<int*>(base + lay.ov), nov, cnt, false, nullptr};
   TRY(launch_probe_and_candidates(st, d_queries, ht->dX, Q, d_knn_ids, d_knn_dists, s, b, false));
   hipLaunchKernelGGL(k_candidates_overflow, dim3(kNaOvGrid), dim3(256), 0, s, ht->dX, d_queries,
-                     Q, D, L, T, K, (const int*)b.qb, (const BucketRec*)ht->recs.p,
-                     (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt,
+                     Q, D, L, T, K, (const uint2*)b.qr, (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt,
                      d_knn_ids, d_knn_dists, b.cnu_q, b.cu_q, (const int*)b.ov,
                      (const int*)b.nov, reinterpret_cast<uint32_t*>(base + lay.ovfix),
                      kNaOvHbits, kNaOvUcap, b.cnt, d_status);

@@ -89,8 +89,10 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
 // matrix in global memory, or the table's slice staged in LDS.  The FP64
 // product of f32 operands is exact, so widening on read is the oracle's
 // arithmetic.  orb is the octet's
-// probe8_octet_bytes(M) of LDS.  Writes qbp[p] (bucket of probe p, or -1)
-// for p < T and *cnu (summed sizes of the matched buckets) when active, and
+// probe8_octet_bytes(M) of LDS.  Writes qrp[p] (member range of probe p's
+// bucket: start, size; (0, 0) when nothing matched), qbp[p] (its bucket, or
+// -1) when qbp is given, for p < T, and *cnu (summed sizes of the matched
+// buckets) when active, and
 // the query's 12-bit coarse-cell key to *cell_key when cell_key is given.
 // Every lane of the octet calls it; T is uniform over the wave.
 template <int J, int MI>
@@ -99,7 +101,8 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
                                    const double* __restrict__ b64, double W,
                                    const uint64_t* __restrict__ seeds, const ProbeIndex& pix,
                                    unsigned char* orb,
-                                   int* __restrict__ qbp, unsigned long long* __restrict__ cnu,
+                                   int* __restrict__ qbp, uint2* __restrict__ qrp,
+                                   unsigned long long* __restrict__ cnu,
                                    unsigned* __restrict__ cell_key = nullptr) {
   const int lane8 = threadIdx.x & 7;
   double* ds = reinterpret_cast<double*>(orb);
@@ -340,15 +343,19 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     const int p = r * 8 + lane8;
     if (active && p < T) {
       long long k = lookup_unverified(key[r], ent[r], pix.dir, pix.shift, pix.recs);
+      uint2 range = make_uint2(0u, 0u);
       if (k >= 0) {
         const unsigned char* line = bucket_line(pix, (uint32_t)k);
-        const uint32_t size = line_size(line);  // in flight with the code loads
-        if (probe_matches(line_codes(line), base, perm, rank, M, pm[r]))
-          sizesum += size;
-        else
+        const uint2 sr = line_range(line);  // in flight with the code loads
+        if (probe_matches(line_codes(line), base, perm, rank, M, pm[r])) {
+          sizesum += sr.y;
+          range = sr;
+        } else {
           k = -1;
+        }
       }
-      qbp[p] = (int)k;
+      if (qbp) qbp[p] = (int)k;
+      qrp[p] = range;
     }
   }
   sizesum = octet_sum_u64(sizesum);
@@ -359,14 +366,15 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
 // in sequence (workgroup b: table b / ceil(Q/32)), so the workgroups in
 // flight look up one or two tables' filters, directories and bucket lines at a
 // time
-// and those stay in cache.  qb[t][q*T + p], cnu_qt[q*L + t].  Needs
+// and those stay in cache.  qr[t][q*T + p], qb[t][q*T + p] (when qb is
+// given), cnu_qt[q*L + t].  Needs
 // probe8_lds_bytes(D, M) bytes of dynamic LDS.
 template <int J, int MI>
 __global__ __launch_bounds__(256) void k_query_probe8(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
     const float* __restrict__ Ad, int Jp, const double* __restrict__ b64, double W,
     const uint64_t* __restrict__ seeds, ProbeIndex pix, int* __restrict__ qb,
-    unsigned long long* __restrict__ cnu_qt) {
+    uint2* __restrict__ qr, unsigned long long* __restrict__ cnu_qt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, oct = tid >> 3;
   const int nqb = (Q + 31) / 32;
@@ -380,7 +388,8 @@ __global__ __launch_bounds__(256) void k_query_probe8(
   const int qa = active ? q : q0;
   probe8_pair<J, MI>(Qm + (long long)qa * D, active, t, D, M, T, Ad + t * M, Jp, b64, W,
                      seeds, pix, orb,
-                     qb + (long long)t * Q * T + (long long)qa * T, cnu_qt + (long long)qa * L + t);
+                     qb ? qb + (long long)t * Q * T + (long long)qa * T : nullptr,
+                     qr + (long long)t * Q * T + (long long)qa * T, cnu_qt + (long long)qa * L + t);
 }
 
 }  // namespace culsh

@@ -23,14 +23,15 @@
 namespace culsh {
 
 // One wave per (query, table).  J = ceil(T / 64) queue slots per lane.
-// qb[t][q*T + p] receives the global bucket index of probe p (or -1);
-// cnu_qt[q*L + t] the summed sizes of the matched buckets.
+// qb[t][q*T + p] receives the global bucket index of probe p (or -1) when
+// qb is given, qr[t][q*T + p] its member range (start, size; (0, 0) when
+// nothing matched); cnu_qt[q*L + t] the summed sizes of the matched buckets.
 template <int J>
 __global__ __launch_bounds__(256) void k_query_probe(
     const float* __restrict__ Qm, int Q, int D, int L, int M, int T,
     const double* __restrict__ At64, const double* __restrict__ b64, double W,
     const uint64_t* __restrict__ seeds, ProbeIndex pix, int* __restrict__ qb,
-    unsigned long long* __restrict__ cnu_qt) {
+    uint2* __restrict__ qr, unsigned long long* __restrict__ cnu_qt) {
   __shared__ double s_ds[4][64];
   __shared__ double s_d[4][128];
   __shared__ int s_perm[4][64];
@@ -212,13 +213,13 @@ __global__ __launch_bounds__(256) void k_query_probe(
       const uint64_t key = make_key(hs, t, pix.tb);
       const uint64_t fb = filter_bits(key);
       long long k = -1;
-      uint32_t size = 0;
+      uint2 range = make_uint2(0u, 0u);
       if ((pix.filter[filter_word(key, pix.tb, tfw)] & fb) == fb)
         k = lookup_unverified(key, pix.dir[key >> pix.shift], pix.dir, pix.shift, pix.recs);
       if (k >= 0) {
         const unsigned char* line = bucket_line(pix, (uint32_t)k);
         const int16_t* bc = line_codes(line);
-        size = line_size(line);
+        range = line_range(line);
         for (int m = 0; m < M; ++m) {
           const int rd = rank[m], ru = rank[M + m];
           int c = sbase[m];
@@ -230,8 +231,10 @@ __global__ __launch_bounds__(256) void k_query_probe(
           }
         }
       }
-      qb[qbBase + p] = (int)k;
-      if (k >= 0) sizesum += size;
+      if (k < 0) range = make_uint2(0u, 0u);
+      if (qb) qb[qbBase + p] = (int)k;
+      qr[qbBase + p] = range;
+      sizesum += range.y;
     }
   }
   sizesum = wave_sum_u64(sizesum);

@@ -65,7 +65,7 @@ namespace {
 long long slot_bytes_for(int L, long long Qcap, long long Pcap, int Rcap) {
   const long long ovf = (long long)culsh_bt::kOvGrid *
                         ((1ll << culsh_bt::kHbitsMax) + 2ll * culsh_bt::kUcapMax) * 4;
-  return Qcap * ((long long)sizeof(BatchQuery) + 8ll * L + 8 + 8 + 4) + 4ll * L * Pcap +
+  return Qcap * ((long long)sizeof(BatchQuery) + 8ll * L + 8 + 8 + 4) + 8ll * L * Pcap +
          (long long)Rcap * (sizeof(BatchReq) + sizeof(BatchClass)) + ovf;
 }
 

@@ -154,9 +154,9 @@ int culsh_st_wait(culsh_st* st);
  * result), but every working buffer lives in one fixed 64 MiB device array
  * of the library (one per GPU, kept until the process ends), so the call
  * allocates nothing and never synchronises.  Limits: the working set of
- * the call (about 20 + 8 L + 4 L T bytes per query, plus 8 MiB) must fit the
+ * the call (about 20 + 8 L + 8 L T bytes per query, plus 8 MiB) must fit the
  * array, else CULSH_E_LIMIT before any launch (c3's L = 32, T = 64: about
- * 6,900 queries); a query with more than 2^17 non-unique candidates or more
+ * 3,500 queries); a query with more than 2^17 non-unique candidates or more
  * than 4096 candidates within the FP32 error bound of its K-th distance
  * gets no results and sets *d_status (a device int the caller provides,
  * cleared by the call) to non-zero.  All calls on one GPU share the array:
