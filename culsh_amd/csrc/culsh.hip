@@ -377,14 +377,12 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
   TRY(k1.ensure(n_all));
   TRY(v0.ensure(n_all));
   TRY(v1.ensure(n_all));
-  TRY(flag.ensure(std::max<long long>(n_all, kSortShards * sort_shard_cap(n_all))));
+  TRY(flag.ensure(n_all));
   TRY(excl.ensure(n_all));
   TRY(scratch.ensure(std::max(rs_scratch_elems(n_all), scan_scratch_elems(n_all)) + 1));
   TRY(dflags.ensure(1));
   DevBuf<uint32_t> bases;
   TRY(bases.ensure(L + 1));
-  DevBuf<unsigned> scnt;
-  TRY(scnt.ensure(kSortShards));
   for (int attempt = 0;; ++attempt) {
     if (attempt > 0) {
       // 64-bit hash collision between two distinct codes: new seeds.
@@ -403,18 +401,18 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
     // Radix passes over the key's table bits and its top hash bits only:
     // with at least log2(N) + 2 hash bits above the lowest sorted bit, runs
     // of equal sorted bits are a bucket's members plus rare collisions, and
-    // k_sort_fixup finishes the order of the full key.
+    // k_sort_inversions / k_sort_runs finish the order of the full key.
     const int hbits = ceil_log2_host(N) + 2;
     const int sb = std::max(0, (64 - ht->tb - hbits) / 8 * 8);
     radix_sort_pairs(&kp, &vp, ka, va, n_all, sb, 64, scratch.p, s);
     if (sb > 0) {
-      // flag (free until k_heads) holds the sharded run lists.
-      const long long cap = sort_shard_cap(n_all);
-      HIP_TRY(hipMemsetAsync(scnt.p, 0, kSortShards * sizeof(unsigned), s));
+      // excl (free until the head scan) holds the bitmap of runs to sort.
+      const long long words = (n_all + 31) / 32;
+      HIP_TRY(hipMemsetAsync(excl.p, 0, (size_t)words * sizeof(uint32_t), s));
       hipLaunchKernelGGL(k_sort_inversions, dim3(grid_for(n_all)), dim3(256), 0, s,
-                         (const uint64_t*)kp, n_all, sb, flag.p, cap, scnt.p);
-      hipLaunchKernelGGL(k_sort_runs, dim3(kSortShards * 8), dim3(256), 0, s, kp, vp, n_all, sb,
-                         (const uint32_t*)flag.p, cap, (const unsigned*)scnt.p);
+                         (const uint64_t*)kp, n_all, sb, excl.p);
+      hipLaunchKernelGGL(k_sort_runs, dim3(grid_for(words)), dim3(256), 0, s, kp, vp, n_all, sb,
+                         (const uint32_t*)excl.p);
     }
     if (kp != k0.p) {
       std::swap(k0.p, k1.p);

@@ -548,17 +548,31 @@ __global__ __launch_bounds__(256) void k_project_f64_tiled(const float* __restri
   }
 }
 
-// Bucket key of every (table, point): keys[t*N + n], vals = n.
+// Bucket key of every (table, point): keys[t*N + n], vals = n.  The
+// workgroup's 256 code rows are contiguous (row i = t*N + n at codes + i*M;
+// 512 M bytes from a 16-byte aligned start) and are staged in LDS with
+// 16-byte loads, so the row reads coalesce; each thread then sums its row's
+// terms from LDS.  M <= 64.
 __global__ __launch_bounds__(256) void k_keys(const int16_t* __restrict__ codes, long long N,
                                               int L, int M, const uint64_t* __restrict__ seeds,
                                               int tb, uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (long long)L * N) return;
+  __shared__ uint4 s_rows[256 * 64 * 2 / 16];
+  int16_t* s16 = reinterpret_cast<int16_t*>(s_rows);
+  const long long n_all = (long long)L * N;
+  const long long i0 = (long long)blockIdx.x * 256;
+  const int rows = (int)(n_all - i0 < 256 ? n_all - i0 : 256);
+  const int nel = rows * M, n16 = nel >> 3;
+  const uint4* src = reinterpret_cast<const uint4*>(codes + i0 * M);
+  for (int c = threadIdx.x; c < n16; c += 256) s_rows[c] = src[c];
+  for (int e = 8 * n16 + threadIdx.x; e < nel; e += 256) s16[e] = codes[i0 * M + e];
+  __syncthreads();
+  if ((int)threadIdx.x >= rows) return;
+  const long long i = i0 + threadIdx.x;
   const int t = (int)(i / N);
   const long long n = i - (long long)t * N;
-  const int16_t* c = codes + i * M;
   const uint64_t seed = seeds[t];
+  const int16_t* c = s16 + threadIdx.x * M;
   uint64_t h = 0;
   for (int m = 0; m < M; ++m) h += code_term(seed, m, (int)c[m]);
   keys[i] = make_key(h, t, tb);

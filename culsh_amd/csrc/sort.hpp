@@ -16,7 +16,7 @@
 namespace culsh {
 
 constexpr int kScanTile = 2048;  // 256 threads x 8
-constexpr int kRsItems = 8;
+constexpr int kRsItems = 16;
 constexpr int kRsTile = 256 * kRsItems;
 
 // Exclusive scan across the 256 threads of a block; `total` gets the sum.
@@ -214,66 +214,57 @@ inline long long rs_tiles(long long n) { return (n + kRsTile - 1) / kRsTile; }
 
 // Completes a stable sort by key bits [shift, 64) into a stable sort by the
 // whole key.  A run of equal key >> shift needs work only when it holds an
-// inversion (a key above its successor): k_sort_inversions lists the first
-// element of each such run (exactly one thread per run lists it: the one at
-// the run's first inversion), then k_sort_runs insertion-sorts each listed
-// run by the full key, stably, one thread per run.  Runs are disjoint, and
-// permuting a run leaves key >> shift unchanged at every position of it.
-// The list is sharded by workgroup (kSortShards shards of sort_shard_cap(n)
-// entries: a shard's workgroups hold at most that many elements) and
-// appended once per wave.
-constexpr int kSortShards = 64;
-inline long long sort_shard_cap(long long n) {
-  const long long blocks = (n + 255) / 256;
-  return (blocks + kSortShards - 1) / kSortShards * 256;
-}
+// inversion (a key above its successor): k_sort_inversions marks the first
+// element of each such run in a bitmap, then k_sort_runs insertion-sorts
+// each marked run by the full key, stably, one thread per run.  Runs are
+// disjoint, and permuting a run leaves key >> shift unchanged at every
+// position of it (a thread that reads the first element of the next run, to
+// find where its own run ends, sees the same key >> shift whatever that
+// run's thread has moved there).
 
+// Thread i at an inversion (keys[i-1] > keys[i], equal key >> shift) finds
+// its run's first element h -- a doubling search backwards, then a binary
+// search (the keys are sorted by key >> shift) -- and sets bit h of `mark`
+// (ceil(n / 32) words, zero on entry).
 __global__ __launch_bounds__(256) void k_sort_inversions(const uint64_t* __restrict__ keys,
                                                          long long n, int shift,
-                                                         uint32_t* __restrict__ list,
-                                                         long long cap,
-                                                         unsigned* __restrict__ count) {
+                                                         uint32_t* __restrict__ mark) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  bool mine = false;
-  long long h = 0;
-  if (i > 0 && i < n) {
-    const uint64_t a = keys[i - 1], b = keys[i];
-    const uint64_t hi = b >> shift;
-    if ((a >> shift) == hi && a > b) {
-      mine = true;
-      h = i - 1;
-      while (h > 0 && (keys[h - 1] >> shift) == hi) {
-        if (keys[h - 1] > keys[h]) {  // an earlier inversion owns the run
-          mine = false;
-          break;
-        }
-        --h;
-      }
+  if (i <= 0 || i >= n) return;
+  const uint64_t a = keys[i - 1], b = keys[i];
+  const uint64_t hi = b >> shift;
+  if ((a >> shift) != hi || a <= b) return;
+  long long l = i - 1, lo = 0;  // keys[l] >> shift == hi; the run starts in [lo, l]
+  for (long long step = 1;; step <<= 1) {
+    const long long c = l - step;
+    if (c < 0) break;
+    if ((keys[c] >> shift) != hi) {
+      lo = c + 1;
+      break;
     }
+    l = c;
   }
-  const unsigned long long m = __ballot(mine);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
-  const int shard = (int)(blockIdx.x % kSortShards);
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(&count[shard], (unsigned)__popcll(m));
-  base = __shfl(base, leader);
-  if (mine)
-    list[(long long)shard * cap + base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] =
-        (uint32_t)h;
+  while (lo < l) {
+    const long long mid = lo + ((l - lo) >> 1);
+    if ((keys[mid] >> shift) < hi)
+      lo = mid + 1;
+    else
+      l = mid;
+  }
+  atomicOr(&mark[lo >> 5], 1u << (lo & 31));
 }
 
-// Grid: kSortShards x parts workgroups; workgroup b takes shard b % kSortShards.
+// Thread j takes word j of the bitmap and sorts the runs whose first
+// elements it marks.
 __global__ __launch_bounds__(256) void k_sort_runs(uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals, long long n,
-                                                   int shift, const uint32_t* __restrict__ list,
-                                                   long long cap,
-                                                   const unsigned* __restrict__ count) {
-  const int shard = (int)(blockIdx.x % kSortShards);
-  const int part = (int)(blockIdx.x / kSortShards), parts = (int)(gridDim.x / kSortShards);
-  const unsigned nruns = count[shard];
-  for (unsigned r = part * 256 + threadIdx.x; r < nruns; r += parts * 256) {
-    const long long h = list[(long long)shard * cap + r];
+                                                   int shift, const uint32_t* __restrict__ mark) {
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= (n + 31) / 32) return;
+  uint32_t w = mark[j];
+  while (w != 0u) {
+    const long long h = j * 32 + (__ffs(w) - 1);
+    w &= w - 1u;
     const uint64_t hi = keys[h] >> shift;
     for (long long a = h + 1; a < n && (keys[a] >> shift) == hi; ++a) {
       const uint64_t k = keys[a];
