@@ -85,14 +85,18 @@ def _roofline(kernels, dom, workload_key=None):
     return out
 
 
-def _search_bytes(D, L, T, K, cnu, cu):
+def _search_bytes(D, L, T, K, cnu, cu, rechecked=0, row_bytes=None):
     """Algorithmic bytes of a search: probe (query read + one key and offset
-    pair per probe) and candidates (bucket id lists, unique rows, results)."""
+    pair per probe) and candidates (bucket id lists, one read of every unique
+    row -- row_bytes, 4 D for the FP32 rows, 2 D for the batches' FP16 copy --
+    the FP32 rows re-read by the FP64 pass, results)."""
     import numpy as np
     cnu = np.asarray(cnu, np.float64)
     cu = np.asarray(cu, np.float64)
     Q = cnu.size
-    return Q * (4.0 * D + 16.0 * L * T), float((4.0 * cnu + 4.0 * D * cu + 8.0 * K).sum())
+    rb = 4.0 * D if row_bytes is None else float(row_bytes)
+    return (Q * (4.0 * D + 16.0 * L * T),
+            float((4.0 * cnu + rb * cu + 8.0 * K).sum()) + 4.0 * D * float(rechecked))
 
 
 def _base(args, value, unit, elapsed, steps, workload, extra_cfg, dtype="f32"):
@@ -215,7 +219,7 @@ def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps):
     assert st.search(Y, Q, X), st.last_error
     assert np.array_equal(st.getKnnIds(), gpu_ids)
     cnu, cu = st.getQueryCandidates()
-    probe_bytes, cand_bytes = _search_bytes(D, L, T, K, cnu, cu)
+    probe_bytes, cand_bytes = _search_bytes(D, L, T, K, cnu, cu, st.getRechecked())
     from culsh_amd import FORM_PRECISION_GROUPED
     cand_kernel = "k_group_candidates" if form == FORM_PRECISION_GROUPED else "k_candidates"
     out = {
@@ -377,15 +381,21 @@ def bench_batch_on(args, c3, steps):
     assert all(p is not None for p in prof), bt.last_error
     stage = {k: sum(p[k] for p in prof) / len(prof) for k in prof[0]}
 
-    # Algorithmic bytes of the batch from each request's candidate counts.
+    # Algorithmic bytes of the batch from each request's candidate counts;
+    # the batch reads the FP16 copy of the rows (D % 8 == 0 and every value
+    # within the half range, as for this data) and re-reads the FP32 rows of
+    # the candidates its FP64 pass re-evaluates.
+    batch_rechecked = bt.getRechecked()
     pb = cb = 0.0
     for Q, K, T, row, off in reqs:
         assert st.reset(ht, K, T)
         assert st.search(Y[row:row + Q], Q, c3.X), st.last_error
         cnu, cu = st.getQueryCandidates()
-        p_, c_ = _search_bytes(cfg.D, cfg.L, T, K, cnu, cu)
+        p_, c_ = _search_bytes(cfg.D, cfg.L, T, K, cnu, cu, 0,
+                               2 * cfg.D if cfg.D % 8 == 0 else 4 * cfg.D)
         pb += p_
         cb += c_
+    cb += 4.0 * cfg.D * batch_rechecked
     sched = bt.get_schedule()
     cand_name = "k_batch_cand" if not sched["fused"] else "k_batch"
     out = {
@@ -399,6 +409,7 @@ def bench_batch_on(args, c3, steps):
                               "runs_ms": [round(r * 1e3, 3) for r in plain_runs]},
         "chosen_vs_plain": round(value / (rows / plain_s), 3),
         "results_equal_plain": bool(same),
+        "fp64_rechecked": int(batch_rechecked),
         "autotune": {"naive_default_ms": round(tune[0], 3), "tuned_ms": round(tune[1], 3),
                      "naive_default": {"probe_queries": 32, "probe_tables": 1,
                                        "queries_per_item": 32, "rows_in_flight": 4,
@@ -413,7 +424,8 @@ def bench_batch_on(args, c3, steps):
     }
     out["roofline"]["note"] = ("kernel time: eager runs of the batch (same kernels, HIP events "
                                "on the launch stream); bytes: candidate-stage algorithmic bytes "
-                               "of all requests (4 C_nu + 4 D C_u + 8 K per query)")
+                               "of all requests (4 C_nu + 2 D C_u + 8 K per query: the FP16 "
+                               "copy of the rows) + 4 D per FP64 re-evaluation")
 
     out["autotune"]["second_mix"] = second_mix
     return out

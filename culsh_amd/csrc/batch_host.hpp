@@ -29,6 +29,7 @@ struct culsh_bt {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   culsh_schedule graph_sched{0, 0, 0, 0, 0, 0};
+  unsigned long long graph_gen = 0;  // the index's data_gen at capture
   bool pending = false;
   unsigned long long rechecked = 0;
 
@@ -138,6 +139,8 @@ BatchArgs batch_args(culsh_bt* bt) {
   a.order = bt->dorder.p;
   a.lkey = bt->dlkey.p;
   a.X = ht->dX;
+  a.X16 = ht->rows16_ok ? ht->dX16.p : nullptr;
+  a.e2 = ht->rows16_e2;
   a.Ad = ht->dAd.p;
   a.b64 = ht->db64.p;
   a.seeds = ht->dseeds.p;
@@ -295,6 +298,7 @@ int bt_capture(culsh_bt* bt) {
   bt->graph = g;
   HIP_TRY(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
   bt->graph_sched = bt->sched;
+  bt->graph_gen = bt->ht->data_gen;
   return CULSH_OK;
 }
 
@@ -323,6 +327,7 @@ int bt_prepare(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d
   if (nreq <= 0 || !reqs) return fail(CULSH_E_ARG, "empty batch");
   if (!dQ || !d_ids || !d_dists) return fail(CULSH_E_ARG, "null device pointer");
   TRY(bt_wait(bt));
+  TRY(ensure_rows16(ht));
   const culsh_schedule& s = bt->sched;
   // Batch order: T descending, then K descending, then request order.
   std::vector<int> ord;
@@ -383,7 +388,10 @@ int bt_prepare(culsh_bt* bt, const culsh_request* reqs, int nreq, const float* d
   item += (vc.back().n + bq - 1) / bq;
   pitem += (vc.back().n + s.probe_queries - 1) / s.probe_queries * ntg;
   TRY(bt_reserve(bt, b, n, p));
-  if (capture && (!bt->exec || !same_schedule(bt->graph_sched, s))) TRY(bt_capture(bt));
+  // A graph holds the index's pointers: capture again after a re-index.
+  if (capture &&
+      (!bt->exec || !same_schedule(bt->graph_sched, s) || bt->graph_gen != ht->data_gen))
+    TRY(bt_capture(bt));
   BatchHdr h;
   std::memset(&h, 0, sizeof(h));
   h.Qm = dQ;

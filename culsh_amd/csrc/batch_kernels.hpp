@@ -83,6 +83,8 @@ struct BatchArgs {
   int* order;       // [Qtot] candidate-stage order (locality within class)
   // Index.
   const float* X;
+  const uint16_t* X16;  // FP16 copy of the rows for the distance pass, or null
+  double e2;            // its bound on ||x - x16||^2
   const float* Ad;
   const double* b64;
   const uint64_t* seeds;
@@ -181,6 +183,8 @@ __global__ __launch_bounds__(256, OCC) void k_batch(BatchArgs a) {
   B.sid = s_pool + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  B.x16 = a.X16;
+  B.e2 = a.e2;
 
   for (;;) {
     __syncthreads();
@@ -392,6 +396,8 @@ __global__ __launch_bounds__(256, 3) void k_batch_cand(BatchArgs a) {
   B.sid = s_pool + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  B.x16 = a.X16;
+  B.e2 = a.e2;
   int shard = (int)(blockIdx.x & 7), tried = 0;
   for (;;) {
     __syncthreads();
@@ -440,6 +446,8 @@ __global__ __launch_bounds__(256, 3) void k_batch_cand_each(BatchArgs a) {
   B.sid = s_pool + kUcap + 2 * kScap;
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_pool + kUcap + 3 * kScap);
+  B.x16 = a.X16;
+  B.e2 = a.e2;
   const BatchQuery d = a.bq[i];
   QueryIO io;
   io.y = hp->Qm + d.qrow * a.D;
@@ -456,7 +464,8 @@ __global__ __launch_bounds__(256, 3) void k_batch_cand_each(BatchArgs a) {
   }
 }
 
-// Queries that overflowed the LDS working set: workgroup b takes list
+// Queries that overflowed the LDS working set (or, with the FP16 rows, its
+// FP64 re-evaluation set): workgroup b takes list
 // entries b, b + G, ...; hash, id list and FP32 distances in global scratch
 // (2^hbits_max + 2 ucap_max words per workgroup).  A query with more than
 // ucap_max non-unique candidates, or more FP64 rechecks than kScapG, sets
@@ -465,7 +474,7 @@ __global__ __launch_bounds__(256) void k_batch_overflow(BatchArgs a, uint32_t* _
                                                         int hbits_max, unsigned ucap_max) {
   __shared__ double s_sd[kScapG];
   __shared__ uint32_t s_sid[kScapG];
-  __shared__ float s_y[kYcap];
+  __shared__ __align__(16) float s_y[kYcap];
   BatchHdr* hp = a.hdr;
   const int n = hp->n_overflow;
   const BatchHdr h = *hp;
@@ -490,6 +499,9 @@ __global__ __launch_bounds__(256) void k_batch_overflow(BatchArgs a, uint32_t* _
     B.sid = s_sid;
     B.scap = kScapG;
     B.ys = s_y;
+    // FP32 rows here: a query whose FP16 band held more candidates than
+    // the LDS path re-evaluates is finished as the search finishes it.
+    B.x16 = nullptr;
     QueryIO io;
     io.y = h.Qm + d.qrow * a.D;
     io.qrq = a.qr + d.poff;

@@ -175,6 +175,96 @@ __device__ inline void l2_fp32_rows(const float* __restrict__ X, const float* __
   }
 }
 
+// FP32 squared L2 of the U candidate rows ids[0..U) of the FP16 copy X16
+// ([N][D] half, D % 8 == 0) against q: each half is widened exactly and the
+// arithmetic is the FP32 path's, so |d32 - d~| <= (D + 3) u d~ for the
+// copy's distance d~.  4 lanes per row, each with the row's 16-byte chunks
+// l, l + 4, ...; R rows per group (64 groups) with their loads issued
+// together; q is read from LDS or L1 per chunk.  Writes d32[u].
+__device__ inline float half_lo(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+}
+__device__ inline float half_hi(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+__device__ inline float sq8(const uint4 x, const float4 qa, const float4 qb, float acc) {
+  float d;
+  d = qa.x - half_lo(x.x); acc = fmaf(d, d, acc);
+  d = qa.y - half_hi(x.x); acc = fmaf(d, d, acc);
+  d = qa.z - half_lo(x.y); acc = fmaf(d, d, acc);
+  d = qa.w - half_hi(x.y); acc = fmaf(d, d, acc);
+  d = qb.x - half_lo(x.z); acc = fmaf(d, d, acc);
+  d = qb.y - half_hi(x.z); acc = fmaf(d, d, acc);
+  d = qb.z - half_lo(x.w); acc = fmaf(d, d, acc);
+  d = qb.w - half_hi(x.w); acc = fmaf(d, d, acc);
+  return acc;
+}
+
+template <int R, int CH>
+__device__ inline void l2_fp16_rows_ch(const uint16_t* __restrict__ X16, const float* __restrict__ q,
+                                       int D, const uint32_t* ids, int U, float* d32) {
+  const int g4 = threadIdx.x >> 2, l4 = threadIdx.x & 3;
+  const int nch = D >> 3;
+  const float4* q4 = reinterpret_cast<const float4*>(q);
+  for (int u0 = g4 * R; u0 < U; u0 += 64 * R) {
+    uint4 xv[R][CH];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint4* xr = reinterpret_cast<const uint4*>(X16 + (long long)ids[min(u0 + r, U - 1)] * D);
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int c = l4 + 4 * k;
+        xv[r][k] = c < nch ? xr[c] : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int c = l4 + 4 * k;
+      if (c < nch) {
+        const float4 qa = q4[2 * c], qb = q4[2 * c + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = sq8(xv[r][k], qa, qb, acc[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      acc[r] += __shfl_xor(acc[r], 2, 4);
+      acc[r] += __shfl_xor(acc[r], 1, 4);
+    }
+    if (l4 == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (u0 + r < U) d32[u0 + r] = acc[r];
+    }
+  }
+}
+
+template <int R>
+__device__ inline void l2_fp16_rows(const uint16_t* __restrict__ X16, const float* __restrict__ q,
+                                    int D, const uint32_t* ids, int U, float* d32) {
+  const int nch = D >> 3;
+  if (nch <= 12) {
+    l2_fp16_rows_ch<R, 3>(X16, q, D, ids, U, d32);
+  } else if (nch <= 16) {
+    l2_fp16_rows_ch<R, 4>(X16, q, D, ids, U, d32);
+  } else {
+    // Wide rows: 16 lanes per row, one chunk per lane at a time.
+    const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    for (int u = g; u < U; u += 16) {
+      const uint4* xr = reinterpret_cast<const uint4*>(X16 + (long long)ids[u] * D);
+      float acc = 0.f;
+      for (int c = lig; c < nch; c += 16) acc = sq8(xr[c], q4[2 * c], q4[2 * c + 1], acc);
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+      if (lig == 0) d32[u] = acc;
+    }
+  }
+}
+
 // Calls ins(id) for every member of every bucket a query matched
 // (cuLSH_SearchTables.cu:321-340).  Wave w takes the (table, probe) entries
 // w*64 + lane + 256 i; the member ranges of kE such rounds are loaded
@@ -251,7 +341,10 @@ __device__ inline unsigned long long query_cnu(int q, int L,
 }
 
 // Exact top-K of query q over its U candidates ids[0..U) given approximate
-// distances dapx[0..U) with |dapx - d| <= rel * d + abs_err.  Selects the
+// distances dapx[0..U) with |dapx - d| <= rel * d + abs_err -- or, when
+// e2 > 0, distances of rows x~ with ||x - x~||^2 <= e2 and
+// |dapx - d~| <= rel * d~ + abs_err for the rows' own distance d~ (the FP16
+// copy; the bound is completed below).  Selects the
 // K-th smallest dapx (radix select over the float bits; distances are >= 0),
 // re-evaluates in FP64 (the oracle's order) every candidate whose dapx can
 // still be at or below the true K-th distance, and applies the reference's
@@ -260,7 +353,8 @@ __device__ inline unsigned long long query_cnu(int q, int L,
 // pass.  All threads of the workgroup must call it.
 __device__ inline bool finish_topk(const float* __restrict__ X, const float* y, int D, int K,
                                    int U, const float* dapx, const uint32_t* ids, double rel,
-                                   double abs_err, double* sd, uint32_t* sid, unsigned scap,
+                                   double abs_err, double e2, double* sd, uint32_t* sid,
+                                   unsigned scap,
                                    int* __restrict__ oid, float* __restrict__ od,
                                    unsigned long long* __restrict__ n_recheck) {
   __shared__ uint32_t s_hist[256];
@@ -301,6 +395,17 @@ __device__ inline bool finish_topk(const float* __restrict__ X, const float* y, 
       __syncthreads();
     }
     const double dk = (double)__uint_as_float(s_prefix);
+    if (e2 > 0.0) {
+      // |sqrt(d~) - sqrt(d)| <= ||x - x~|| = e, so |d~ - d| <= 2 e sqrt(d) + e^2
+      // <= delta d + e^2 (1 + 1 / delta) for every delta > 0; delta =
+      // sqrt(e2 / dk) (at most 1/4) keeps the band near the K-th distance
+      // narrow.  With |dapx - d~| <= rel d~ + abs_err this gives
+      // |dapx - d| <= rel' d + abs' for the rel', abs' below.
+      const double delta = dk > e2 * 16.0 ? sqrt(e2 / dk) : 0.25;
+      const double ext = e2 * (1.0 + 1.0 / delta);
+      abs_err = (abs_err + ext * (1.0 + rel)) * (1.0 + 0x1.0p-40);
+      rel = (rel + delta * (1.0 + rel)) * (1.0 + 0x1.0p-40);
+    }
     thr = ((dk + abs_err) * ((1.0 + rel) / (1.0 - rel)) + abs_err) * (1.0 + 1e-12) +
           (double)D * 0x1.0p-120;
   } else {
@@ -428,6 +533,11 @@ struct CandBufs {
   uint32_t* sid;    // their ids
   unsigned scap;
   float* ys;        // query vector staging (kYcap floats, may alias hash), or null
+  // Optional FP16 copy of the dataset rows ([N][D] IEEE half, D % 8 == 0)
+  // with e2 >= max over rows of ||x - x16||^2: the distance pass then reads
+  // the copy (half the bytes) and the FP64 pass the FP32 rows.
+  const uint16_t* x16 = nullptr;
+  double e2 = 0.0;
   // Optional [3] shader-clock totals of the query's phases: union (member
   // expansion + dedup), FP32 distances, top-K (selection, FP64 recheck,
   // sort, tie rule); one atomic add per phase per query, or null.
@@ -537,7 +647,10 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   if (y_staged)
     for (int d = tid; d < D; d += 256) B.ys[d] = y[d];
   __syncthreads();
-  if (y_staged)
+  const bool half = B.x16 != nullptr && y_staged;
+  if (half)  // half the bytes per row: half the rows per lane group for as many in flight
+    l2_fp16_rows<(R > 1 ? R / 2 : 1)>(B.x16, B.ys, D, B.ulist, U, B.d32);
+  else if (y_staged)
     l2_fp32_rows<R>(X, B.ys, D, B.ulist, U, B.d32);
   else
     l2_fp32_rows<R>(X, y, D, B.ulist, U, B.d32);
@@ -547,8 +660,9 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   }
   // Direct FP32: |d32 - d| <= (D + 3) u d; margin 2 (Higham ch. 3).
   const double rel = 2.0 * (double)(D + 8) * kU32d;
-  const bool done = finish_topk(X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0, B.sd,
-                                B.sid, B.scap, io.oid, io.od, n_recheck);
+  const bool done = finish_topk(X, y_staged ? B.ys : y, D, K, U, B.d32, B.ulist, rel, 0.0,
+                                half ? B.e2 : 0.0, B.sd, B.sid, B.scap, io.oid, io.od,
+                                n_recheck);
   phase_end(2);
   if (!done) return false;
   if (tid == 0) {
@@ -624,7 +738,7 @@ __global__ __launch_bounds__(256) void k_candidates_global(
     unsigned long long* __restrict__ phase) {
   __shared__ double s_sd[kScapG];
   __shared__ uint32_t s_sid[kScapG];
-  __shared__ float s_y[kYcap];
+  __shared__ __align__(16) float s_y[kYcap];
   uint32_t* base = scratch + (size_t)blockIdx.x * ((1ull << hbits) + 2ull * ucap);
   CandBufs B;
   B.hash = base;
@@ -666,7 +780,7 @@ __global__ __launch_bounds__(256) void k_candidates_overflow(
     unsigned ucap_max, unsigned long long* __restrict__ n_recheck, int* __restrict__ err) {
   __shared__ double s_sd[kScapG];
   __shared__ uint32_t s_sid[kScapG];
-  __shared__ float s_y[kYcap];
+  __shared__ __align__(16) float s_y[kYcap];
   const int n = *n_list;
   uint32_t* base = scratch + (size_t)blockIdx.x * ((1ull << hbits_max) + 2ull * ucap_max);
   for (int e = blockIdx.x; e < n; e += gridDim.x) {
@@ -920,7 +1034,7 @@ __global__ __launch_bounds__(256, 3) void k_group_candidates(
       const double qb2 = (double)s_qnorm[g] + (double)xmax;
       const double abs_err = 2.0 * (double)(D + 4) * kU32d * qb2 * qb2 * (1.0 + 1e-6);
       const bool ok = finish_topk(X, Qm + (long long)q * D, D, K, (int)cu[g], qd + s_qoff[g],
-                                  qi + s_qoff[g], 0.0, abs_err, s_sd, s_sid, kScap,
+                                  qi + s_qoff[g], 0.0, abs_err, 0.0, s_sd, s_sid, kScap,
                                   out_ids + (long long)q * K, out_d + (long long)q * K,
                                   n_recheck);
       if (tid == 0) {

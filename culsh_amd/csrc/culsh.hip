@@ -151,6 +151,13 @@ struct culsh_ht {
   const float* dX = nullptr;          // resident dataset [N][D]
   const float* hX_indexed = nullptr;  // host pointer index() was given
   unsigned long long data_gen = 0;    // generation of the resident dataset
+  // FP16 copy of the resident rows for the batching form's distance pass
+  // (built on first use per dataset generation; rows16_ok false when D % 8
+  // != 0 or a value leaves the half range) and its bound on ||x - x16||^2.
+  DevBuf<uint16_t> dX16;
+  unsigned long long rows16_gen = 0;
+  bool rows16_ok = false;
+  double rows16_e2 = 0.0;
   DevBuf<BucketRec> recs;             // [B_total] sorted by key: key, start, size
   DevBuf<uint32_t> sids;              // [L*N] members, bucket by bucket
   DevBuf<int16_t> bcodes;             // [B_total][M]
@@ -184,6 +191,8 @@ struct culsh_ht {
     data_gen = next_data_gen();
     dX_own.release();
     dX = nullptr;
+    dX16.release();
+    rows16_ok = false;
     hX_indexed = nullptr;
     recs.release();
     sids.release();
@@ -521,6 +530,37 @@ int project_dataset(culsh_ht* ht, DevBuf<int16_t>& codes, FILE* dbg) {
   if (herr)
     return fail(CULSH_E_RANGE, "a hash code falls outside [-32766, 32766] (W too small for the data?)");
   if (dbg) fprintf(dbg, "[HashTables::index]\tprojected %lld x %d, %llu FP64 rechecks\n", N, J, count);
+  return CULSH_OK;
+}
+
+// The FP16 copy of the resident rows for the current dataset generation
+// (one conversion pass and one host synchronisation on first use).
+int ensure_rows16(culsh_ht* ht) {
+  if (ht->rows16_gen == ht->data_gen) return CULSH_OK;
+  ht->rows16_ok = false;
+  ht->dX16.release();
+  ht->rows16_gen = ht->data_gen;
+  if (!ht->dX || ht->D % 8 != 0) return CULSH_OK;
+  hipStream_t s = ht->stream;
+  const long long N = ht->N;
+  TRY(ht->dX16.exact((size_t)N * ht->D));
+  DevBuf<unsigned long long> e2;
+  TRY(e2.ensure(1));
+  HIP_TRY(hipMemsetAsync(e2.p, 0, 8, s));
+  hipLaunchKernelGGL(k_rows_f16, dim3(grid_for(N, 16)), dim3(256), 0, s, ht->dX, N, ht->D,
+                     ht->dX16.p, e2.p);
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, e2.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  double v;
+  std::memcpy(&v, &bits, 8);
+  if (std::isfinite(v)) {
+    ht->rows16_ok = true;
+    ht->rows16_e2 = v;
+  } else {
+    ht->dX16.release();
+  }
   return CULSH_OK;
 }
 

@@ -32,6 +32,35 @@ __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, 
   if (n < N && lig == 0) nrm[n] = (float)(sqrt(acc) * (1.0 + 0x1.0p-20));
 }
 
+// FP16 copy of the rows (round to nearest even) and the largest squared row
+// error ||x - x16||^2 over the rows (FP64 sum, rounded up), kept as the bits
+// of a non-negative double, whose integer order is its numeric order; a
+// value that leaves the half range makes the error infinite or NaN, and the
+// caller then keeps to the FP32 rows.  16 lanes per row.
+__global__ __launch_bounds__(256) void k_rows_f16(const float* __restrict__ X, long long N, int D,
+                                                  uint16_t* __restrict__ x16,
+                                                  unsigned long long* __restrict__ e2max_bits) {
+  const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
+  const long long n = (long long)blockIdx.x * 16 + g;
+  double acc = 0.0;
+  if (n < N) {
+    const float* x = X + n * D;
+    uint16_t* h = x16 + n * D;
+    for (int d = lig; d < D; d += 16) {
+      const _Float16 v = (_Float16)x[d];
+      h[d] = __builtin_bit_cast(uint16_t, v);
+      const double e = (double)x[d] - (double)(float)v;
+      acc = fma(e, e, acc);
+    }
+  }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+  if (n < N && lig == 0) {
+    const double up = acc * (1.0 + 0x1.0p-30);  // the sum's rounding, D < 2^22
+    atomicMax(e2max_bits, (unsigned long long)__double_as_longlong(up != up ? INFINITY : up));
+  }
+}
+
 // FP32 squared norm of each row (the norm expansion's |x|^2) and the largest
 // row norm (as float bits; norms are non-negative so integer max orders
 // them).
@@ -97,10 +126,14 @@ __device__ inline void store_code_tile(const int16_t* sc, long long n0, int j0, 
     const int ms = max(j0 - t * M, 0), me = min(jend - t * M, M);  // m range in the tile
     const int cs = t * M + ms - j0;                                // its first tile column
     int16_t* dst = codes + ((long long)t * N + n0) * M;
+    // e / n for e < 2^12, n <= 64 as floor((e + 0.5) / n) in FP32: the
+    // quotient is at least 1/128 away from an integer, far beyond the
+    // rounding error.
     if ((M & 1) == 0 && (ms & 1) == 0 && (me & 1) == 0) {
       const int np = (me - ms) >> 1;
+      const float rn = 1.0f / (float)np;
       for (int e = tid; e < rows * np; e += 256) {
-        const int r = e / np, q = e - r * np;
+        const int r = (int)(((float)e + 0.5f) * rn), q = e - r * np;
         const int c = cs + 2 * q;
         const uint32_t v = (uint32_t)(uint16_t)sc[r * kPjCols + c] |
                            ((uint32_t)(uint16_t)sc[r * kPjCols + c + 1] << 16);
@@ -108,8 +141,9 @@ __device__ inline void store_code_tile(const int16_t* sc, long long n0, int j0, 
       }
     } else {
       const int nm = me - ms;
+      const float rn = 1.0f / (float)nm;
       for (int e = tid; e < rows * nm; e += 256) {
-        const int r = e / nm, q = e - r * nm;
+        const int r = (int)(((float)e + 0.5f) * rn), q = e - r * nm;
         dst[(long long)r * M + ms + q] = sc[r * kPjCols + cs + q];
       }
     }

@@ -437,6 +437,12 @@ int culsh_srv_create(culsh_ht* const* indexes, int nindex, const culsh_srv_confi
   int maxL = 0;
   for (auto* h : srv->ht) maxL = std::max(maxL, h->L);
   if (cfg->batching) {
+    // The indexes' FP16 row copies (the batches' distance pass) come first,
+    // so the slots are sized from what is left.
+    for (auto* h : srv->ht) {
+      const int rc = ensure_rows16(h);
+      if (rc != CULSH_OK) return bail(rc);
+    }
     // Initial slot capacity from the free HBM now: Q queries and 32 Q probe
     // slots per table.  Slots grow later by the free HBM at that moment.
     size_t freeb = 0, totalb = 0;

@@ -215,3 +215,52 @@ def test_batch_autotune_keeps_results():
     assert r[1] <= r[0]
     ids2, d2 = _run_batch(bt, reqs, dY, total)
     _check(reqs, ids2, d2, _alone(ht, Y, reqs))
+
+
+def test_batch_after_reindex_searches_the_new_index():
+    """A plan whose graph was captured on one index build searches the next
+    build of the same HashTables (another dataset of the same shape): the
+    graph is captured again, with the new index's buffers and FP16 rows."""
+    import torch
+    print("batch after re-index", flush=True)
+    X, Y, A, b, ht, reqs, total, dY = _setup(seed=261, n=10, qmax=120)
+    bt = Batch(ht)
+    ids, dists = _run_batch(bt, reqs, dY, total)
+    _check(reqs, ids, dists, _alone(ht, Y, reqs))
+    X2 = wl.mixture(X.shape[0], X.shape[1], 30, 0.2, 262)
+    Y2 = wl.queries_near(X2, Y.shape[0], 0.05, 263)
+    assert ht.index(X2), ht.last_error
+    ids2, d2 = _run_batch(bt, reqs, torch.from_numpy(Y2).cuda(), total)
+    _check(reqs, ids2, d2, _alone(ht, Y2, reqs))
+
+
+@pytest.mark.parametrize("case", ["d_not_multiple_of_8", "beyond_half_range", "coarse_half"])
+def test_batch_distance_rows(case):
+    """The batch's distance pass reads an FP16 copy of the rows when D is a
+    multiple of 8 and every value fits a half; results stay identical to the
+    FP32 search: with D = 20 (FP32 rows), with values beyond the half range
+    (FP32 rows), and with values near 1000, whose half spacing (0.5) is
+    large next to the distances (wide bands of FP64 re-evaluation)."""
+    import torch
+    print("batch distance rows", case, flush=True)
+    rng = np.random.default_rng(271)
+    D = 20 if case == "d_not_multiple_of_8" else 24
+    X = wl.mixture(8000, D, 20, 0.2, 272)
+    if case == "beyond_half_range":
+        X = (X * 1.0e5).astype(np.float32)
+    elif case == "coarse_half":
+        X = (X + 1000.0).astype(np.float32)
+    W = {"d_not_multiple_of_8": 1.5, "beyond_half_range": 1.5e5, "coarse_half": 1.5}[case]
+    L, M = 5, 8
+    shapes = _mixed_shapes(rng, 12, 150)
+    reqs, rows, total = layout_requests(shapes)
+    Y = wl.queries_near(X, rows, 0.05 * (1.0e5 if case == "beyond_half_range" else 1.0), 273)
+    A, b = wl.projection(L, D, M, W, 274)
+    ht = _index(X, A, b, L, M, W)
+    bt = Batch(ht)
+    ids, dists = _run_batch(bt, reqs, torch.from_numpy(Y).cuda(), total)
+    _check(reqs, ids, dists, _alone(ht, Y, reqs))
+    Q, K, T, row, off = next(r for r in reqs if r[0] > 0)
+    o = run_oracle(X, Y[row:row + Q], A, b, L, M, W, K, T, want_buckets=False)
+    np.testing.assert_array_equal(ids[off:off + Q * K].reshape(Q, K), o["ids"])
+    np.testing.assert_array_equal(dists[off:off + Q * K].reshape(Q, K), o["dists"])
