@@ -616,8 +616,11 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
   for (uint32_t i = tid; i < hslots; i += 256) B.hash[i] = kEmpty;
   __syncthreads();
   if (cnu > 0) {
+    // Hash insert; the lanes that inserted a new id append it to the list
+    // with one counter update per wave.
     for_each_member(io.qrq, io.tstride, L, io.T, sids, [&](uint32_t id) {
       uint32_t h = (id * 0x9E3779B1u) >> (32 - hbits);
+      bool fresh = false;
       for (uint32_t probe = 0;; ++probe) {
         if (probe == hslots) {
           s_ovf = 1;
@@ -625,15 +628,25 @@ __device__ inline bool query_candidates(const QueryIO& io, const float* __restri
         }
         const uint32_t old = atomicCAS(&B.hash[h], kEmpty, id);
         if (old == kEmpty) {
-          const unsigned idx = atomicAdd(&s_count, 1u);
-          if (idx < B.ucap)
-            B.ulist[idx] = id;
-          else
-            s_ovf = 1;
+          fresh = true;
           break;
         }
         if (old == id) break;
         h = (h + 1) & hmask;
+      }
+      const unsigned long long bm = __ballot(fresh);
+      if (bm != 0ull) {
+        const int lane = threadIdx.x & 63, leader = __ffsll((long long)bm) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&s_count, (unsigned)__popcll(bm));
+        base = __shfl(base, leader);
+        if (fresh) {
+          const unsigned idx = base + (unsigned)__popcll(bm & ((1ull << lane) - 1ull));
+          if (idx < B.ucap)
+            B.ulist[idx] = id;
+          else
+            s_ovf = 1;
+        }
       }
     });
   }

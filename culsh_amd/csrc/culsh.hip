@@ -451,13 +451,13 @@ int build_buckets(culsh_ht* ht, const DevBuf<int16_t>& codes, FILE* dbg) {
     if (!collision) break;
     if (dbg) fprintf(dbg, "[HashTables::index]\tbucket key collision, reseeding\n");
   }
-  TRY(build_probe_index(ht));
   // The sorted member ids become the index's id array.
   ht->sids.release();
   ht->sids.p = v0.p;
   ht->sids.n = v0.n;
   v0.p = nullptr;
   v0.n = 0;
+  TRY(build_probe_index(ht));
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipGetLastError());
   return CULSH_OK;

@@ -259,7 +259,10 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     int size = 1;
     // Pop p = 8 r + j runs with r fixed at compile time, so the popped set
     // goes to register r of lane j and, while every earlier pop made a new
-    // entry (size == p), so does the new shift entry (slot p).
+    // entry (size == p), so does the new shift entry (slot p).  Before pop p
+    // at most p slots (0 .. p - 1) hold entries and the new one goes to slot
+    // size <= p, so registers above r are empty: the scan and the writes
+    // stop at register r.
 #pragma unroll
     for (int r = 0; r < J; ++r) {
       for (int j = (r == 0 ? 1 : 0); j < 8; ++j) {
@@ -270,7 +273,7 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
         Code probe = 0;
         int bi = 0;
 #pragma unroll
-        for (int i = 0; i < J; ++i)
+        for (int i = 0; i <= r; ++i)
           if (qs[i] < omin) {
             omin = qs[i];
             bi = i;
@@ -288,7 +291,7 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
           const Code ps = pe - ((Code)1 << maxPos);
           if (lane8 == wl) {
 #pragma unroll
-            for (int i = 0; i < J; ++i)
+            for (int i = 0; i <= r; ++i)
               if (i == wi) {
                 qc[i] = pe;
                 qs[i] = (uint64_t)__double_as_longlong(e);
@@ -301,7 +304,7 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
             }
           } else if (lane8 == (size & 7)) {
 #pragma unroll
-            for (int i = 0; i < J; ++i)
+            for (int i = 0; i <= r; ++i)
               if (i == (size >> 3)) {
                 qc[i] = ps;
                 qs[i] = (uint64_t)__double_as_longlong(sh);
