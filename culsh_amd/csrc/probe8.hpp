@@ -58,8 +58,9 @@ __device__ inline bool probe_matches(const int16_t* __restrict__ bc, const int* 
 }
 
 // Key sum of probe `mask`: the base key sum with the moved dimensions'
-// terms replaced.
-__device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int* base,
+// terms replaced; dt[di] is the change of the sum when delta di is applied
+// (di = m: -1 on m, M + m: +1 on m).
+__device__ inline uint64_t probe_hash(uint64_t basesum, const uint64_t* dt,
                                       const signed char* perm, const unsigned char* rank, int M,
                                       uint64_t mask) {
   uint64_t hs = basesum;
@@ -68,11 +69,9 @@ __device__ inline uint64_t probe_hash(uint64_t basesum, uint64_t seed, const int
     const int i = __ffsll((unsigned long long)mm) - 1;
     mm &= mm - 1;
     const int di = perm[i];
-    const int dim = di < M ? di : di - M;
     const int pr = rank[di < M ? di + M : di - M];
     if (pr != 255 && ((mask >> pr) & 1ull)) continue;
-    const int b0 = base[dim];
-    hs += code_term(seed, dim, b0 + (di >= M ? 1 : -1)) - code_term(seed, dim, b0);
+    hs += dt[di];
   }
   return hs;
 }
@@ -316,6 +315,22 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
     }
   }
 
+  // Key sum changes of the 2M deltas, in the sorted deltas' slots (the
+  // queue no longer reads them).
+  uint64_t* dt = reinterpret_cast<uint64_t*>(ds);
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < MI; ++i) {
+    const int m = lane8 + 8 * i;
+    if (T > 1 && valid[i]) {
+      const int b0 = base[m];
+      const uint64_t t0 = code_term(seed, m, b0);
+      dt[m] = code_term(seed, m, b0 - 1) - t0;
+      dt[M + m] = code_term(seed, m, b0 + 1) - t0;
+    }
+  }
+  wave_lds_sync();
+
   // Keys, lookup, verify: lane l handles probes l + 8r.  Each probe's key
   // filter word (in cache) is loaded first; only the keys the filter passes
   // load their directory entry, and a directory hit its bucket line.  All of
@@ -327,7 +342,7 @@ __device__ __forceinline__ void probe8_pair(const float* y, bool active, int t, 
   for (int r = 0; r < J; ++r) {
     const int p = r * 8 + lane8;
     key[r] = (active && p < T)
-                 ? make_key(probe_hash(basesum, seed, base, perm, rank, M, pm[r]), t, pix.tb)
+                 ? make_key(probe_hash(basesum, dt, perm, rank, M, pm[r]), t, pix.tb)
                  : 0ull;
   }
 #pragma unroll
