@@ -221,6 +221,9 @@ typedef struct {
                           persistent candidate kernel */
 } culsh_schedule;
 
+/* A plan for batches against `ht`.  The first batch on a dataset builds an
+ * FP16 copy of its rows (2 N D bytes, held by the index) for the batches'
+ * distance pass; results are unchanged.  No counterpart in the reference. */
 int culsh_bt_create(culsh_ht* ht, culsh_bt** out);
 void culsh_bt_destroy(culsh_bt* bt);
 int culsh_bt_set_schedule(culsh_bt* bt, const culsh_schedule* s);
