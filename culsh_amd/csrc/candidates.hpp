@@ -270,7 +270,8 @@ __device__ inline void l2_fp16_rows(const uint16_t* __restrict__ X16, const floa
 // w*64 + lane + 256 i; the member ranges of kE such rounds are loaded
 // together, then per round the wave prefix-sums the bucket sizes and spreads
 // the member ids over its lanes, kR rounds of 64 members with their id loads
-// in flight together.  qrq[t * tstride + p] is the member range (start,
+// in flight together (and in flight while the previous batch is inserted).
+// qrq[t * tstride + p] is the member range (start,
 // size; size 0 when nothing matched) of the query's probe p of table t.
 // All threads of the workgroup must call it.
 template <class Ins>
@@ -279,6 +280,10 @@ __device__ inline void for_each_member(const uint2* __restrict__ qrq, long long 
   constexpr int kE = 8, kR = 8;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int LT = L * T;
+  // The ids of one batch of kR rounds are inserted after the next batch's
+  // loads are issued, so consecutive batches' loads overlap the inserts.
+  uint32_t pend[kR];
+  bool has_pend = false;  // uniform over the wave
   for (int p0 = w * 64; p0 < LT; p0 += 256 * kE) {
     uint32_t st[kE], sz[kE];
 #pragma unroll
@@ -325,11 +330,21 @@ __device__ inline void for_each_member(const uint2* __restrict__ qrq, long long 
             if (m < tot) id[r] = sids[st_l + (m - (inc_l - sz_l))];
           }
         }
+        if (has_pend) {
 #pragma unroll
-        for (int r = 0; r < kR; ++r)
-          if (id[r] != kEmpty) ins(id[r]);
+          for (int r = 0; r < kR; ++r)
+            if (pend[r] != kEmpty) ins(pend[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) pend[r] = id[r];
+        has_pend = true;
       }
     }
+  }
+  if (has_pend) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r)
+      if (pend[r] != kEmpty) ins(pend[r]);
   }
 }
 
