@@ -543,9 +543,14 @@ int ensure_rows16(culsh_ht* ht) {
   if (!ht->dX || ht->D % 8 != 0) return CULSH_OK;
   hipStream_t s = ht->stream;
   const long long N = ht->N;
-  TRY(ht->dX16.exact((size_t)N * ht->D));
   DevBuf<unsigned long long> e2;
-  TRY(e2.ensure(1));
+  if (ht->dX16.exact((size_t)N * ht->D) != CULSH_OK || e2.ensure(1) != CULSH_OK) {
+    // The copy only saves bytes: without the memory for it the batches
+    // read the FP32 rows.
+    ht->dX16.release();
+    (void)hipGetLastError();
+    return CULSH_OK;
+  }
   HIP_TRY(hipMemsetAsync(e2.p, 0, 8, s));
   hipLaunchKernelGGL(k_rows_f16, dim3(grid_for(N, 16)), dim3(256), 0, s, ht->dX, N, ht->D,
                      ht->dX16.p, e2.p);
