@@ -714,8 +714,9 @@ __global__ __launch_bounds__(256, 3) void k_candidates(
     int* __restrict__ out_ids, float* __restrict__ out_d, unsigned long long* __restrict__ cnu_q,
     unsigned long long* __restrict__ cu_q, int* __restrict__ overflow,
     int* __restrict__ n_overflow, unsigned long long* __restrict__ n_recheck,
-    const uint32_t* __restrict__ order, unsigned long long* __restrict__ phase) {
-  __shared__ uint32_t s_hash[kHcap];
+    const uint32_t* __restrict__ order, unsigned long long* __restrict__ phase,
+    const uint16_t* __restrict__ x16, double e2) {
+  __shared__ __align__(16) uint32_t s_hash[kHcap];
   __shared__ uint32_t s_u[kUcap];
   static_assert(kUcap + 3 * kScap + kYcap <= kHcap, "post-union LDS layout");
   CandBufs B;
@@ -729,6 +730,8 @@ __global__ __launch_bounds__(256, 3) void k_candidates(
   B.scap = kScap;
   B.ys = reinterpret_cast<float*>(s_hash + kUcap + 3 * kScap);
   B.phase = phase;
+  B.x16 = x16;
+  B.e2 = e2;
   int q = blockIdx.x;
   if (order) {
     const int per = (int)(gridDim.x >> 3);  // grid is a multiple of 8

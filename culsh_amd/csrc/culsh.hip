@@ -1366,10 +1366,15 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
       TRY(locality_order(st, dQ, Q, s, &order));
       grid = (unsigned)((Q + 7) / 8 * 8);
     }
+    // The precision form's distance pass reads the resident rows' FP16 copy
+    // when it exists (results unchanged; the FP64 pass reads the FP32 rows).
+    const bool half = st->form == CULSH_FORM_PRECISION && dX == ht->dX && ht->rows16_ok &&
+                      ht->rows16_gen == ht->data_gen;
     if (timed) HIP_TRY(hipEventRecord(st->ev[3], s));
     hipLaunchKernelGGL(k_candidates, dim3(grid), dim3(256), 0, s, dX, dQ, Q, D, L, T, K,
                        (const uint2*)b.qr, (const uint32_t*)ht->sids.p, (const unsigned long long*)b.cnu_qt, d_ids,
-                       d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order, b.phase);
+                       d_dists, b.cnu_q, b.cu_q, b.ov, b.nov, b.cnt, order, b.phase,
+                       half ? (const uint16_t*)ht->dX16.p : nullptr, ht->rows16_e2);
   }
   if (timed) HIP_TRY(hipEventRecord(st->ev[2], s));
   HIP_TRY(hipGetLastError());
@@ -1391,6 +1396,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   // working sets on the host.
   const bool async = !wait && st->form != CULSH_FORM_PRECISION_GROUPED;
   culsh_ht* ht = st->ht;
+  if (st->form == CULSH_FORM_PRECISION && dX == ht->dX) TRY(ensure_rows16(ht));
   const int L = ht->L, D = ht->D, T = st->T, K = st->K;
   const long long QT = (long long)Q * T;
   if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
