@@ -58,8 +58,12 @@ enum {
  * K-th distance. */
 enum {
   CULSH_FORM_PLAIN = 0,      /* FP32 projection (MFMA), direct FP32 distances */
-  CULSH_FORM_PRECISION = 1,  /* BF16x3 split-precision projection (MFMA); direct
-                                FP32 distances */
+  CULSH_FORM_PRECISION = 1,  /* BF16x3 split-precision projection (MFMA);
+                                searches of the resident dataset compute the
+                                distance pass on an FP16 copy of its rows
+                                (2 N D bytes, built with one synchronization by
+                                the first such search when D % 8 == 0 and the
+                                values fit a half), others direct FP32 */
   CULSH_FORM_PRECISION_GROUPED = 2  /* search only: distances by the norm
                                 expansion |q|^2 + |x|^2 - 2 q.x on MFMA for groups
                                 of 16 locality-ordered queries (pays only when
@@ -163,8 +167,9 @@ int culsh_st_wait(culsh_st* st);
  * calls that overlap on the device race (order them on one stream). */
 int culsh_st_search_device_noalloc(culsh_st* st, const float* d_queries, int Q, int* d_knn_ids,
                                    float* d_knn_dists, int* d_status, void* stream);
-/* Distance form of search(): CULSH_FORM_PLAIN (default), _PRECISION (same
- * direct distances) or _PRECISION_GROUPED (grouped norm expansion). */
+/* Distance form of search(): CULSH_FORM_PLAIN (default), _PRECISION (the
+ * FP16 distance pass on the resident dataset) or _PRECISION_GROUPED (grouped
+ * norm expansion). */
 int culsh_st_set_form(culsh_st* st, int form);
 
 /* Results and metrics (cuLSH_SearchTables.cu:83-105). */
