@@ -552,8 +552,8 @@ int ensure_rows16(culsh_ht* ht) {
     return CULSH_OK;
   }
   HIP_TRY(hipMemsetAsync(e2.p, 0, 8, s));
-  hipLaunchKernelGGL(k_rows_f16, dim3(grid_for(N, 16)), dim3(256), 0, s, ht->dX, N, ht->D,
-                     ht->dX16.p, e2.p);
+  hipLaunchKernelGGL(k_rows_f16, dim3(std::min<unsigned>(grid_for(N, 16), 4096u)), dim3(256), 0,
+                     s, ht->dX, N, ht->D, ht->dX16.p, e2.p);
   unsigned long long bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, e2.p, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

@@ -36,28 +36,41 @@ __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, 
 // error ||x - x16||^2 over the rows (FP64 sum, rounded up), kept as the bits
 // of a non-negative double, whose integer order is its numeric order; a
 // value that leaves the half range makes the error infinite or NaN, and the
-// caller then keeps to the FP32 rows.  16 lanes per row.
+// caller then keeps to the FP32 rows.  16 lanes per row, rows grid-strided;
+// one atomic per workgroup.
 __global__ __launch_bounds__(256) void k_rows_f16(const float* __restrict__ X, long long N, int D,
                                                   uint16_t* __restrict__ x16,
                                                   unsigned long long* __restrict__ e2max_bits) {
+  __shared__ double s_max[4];
   const int g = threadIdx.x >> 4, lig = threadIdx.x & 15;
-  const long long n = (long long)blockIdx.x * 16 + g;
-  double acc = 0.0;
-  if (n < N) {
+  double mx = 0.0;
+  for (long long n = (long long)blockIdx.x * 16 + g; n < N; n += (long long)gridDim.x * 16) {
     const float* x = X + n * D;
     uint16_t* h = x16 + n * D;
+    double acc = 0.0;
     for (int d = lig; d < D; d += 16) {
       const _Float16 v = (_Float16)x[d];
       h[d] = __builtin_bit_cast(uint16_t, v);
       const double e = (double)x[d] - (double)(float)v;
       acc = fma(e, e, acc);
     }
-  }
 #pragma unroll
-  for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
-  if (n < N && lig == 0) {
+    for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
     const double up = acc * (1.0 + 0x1.0p-30);  // the sum's rounding, D < 2^22
-    atomicMax(e2max_bits, (unsigned long long)__double_as_longlong(up != up ? INFINITY : up));
+    mx = (up != up || mx != mx) ? (double)NAN : fmax(mx, up);
+  }
+  // Largest over the workgroup (NaN wins), then one atomic.
+#pragma unroll
+  for (int off = 16; off < 64; off <<= 1) {
+    const double o = __shfl_xor(mx, off);
+    mx = (o != o || mx != mx) ? (double)NAN : fmax(mx, o);
+  }
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = s_max[0];
+    for (int w = 1; w < 4; ++w) m = (s_max[w] != s_max[w] || m != m) ? (double)NAN : fmax(m, s_max[w]);
+    atomicMax(e2max_bits, (unsigned long long)__double_as_longlong(m != m ? INFINITY : m));
   }
 }
 
