@@ -17,6 +17,10 @@ Default (--config all), the metric on its own workloads, all on one GPU:
                     candidate kernel (k_batch_cand) timed with HIP events on
                     its stream in eager runs of the same batch
   c3_plain          the c3 search (Q=50,000, K=100, T=64), plain form
+  c3_precision      the same in the precision form (BF16x3 index projection,
+                    FP16-row distance pass, FP64 rechecks): rechecked
+                    fractions, ids and distances of 300 queries against the
+                    oracle, distance error against FP64
   c5_serve          serving form: two resident indexes (2M x 128 and
                     1M x 960), Poisson streams at 2k-20k requests/s, Q
                     log-uniform in [1, 512], 70/30 over the indexes,
@@ -181,8 +185,10 @@ class C3Index:
         return self._oracle
 
 
-def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps):
-    """Device search of the workload's Q queries, `steps` times back to back."""
+def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps, traffic_key=None):
+    """Device search of the workload's Q queries, `steps` times back to back;
+    `traffic_key` names the measured-traffic entry of profiles/traffic.json
+    (default: the workload and --form)."""
     import numpy as np
     import torch
 
@@ -232,9 +238,73 @@ def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps):
         "roofline": _roofline({"k_query_probe8": (probe_ms / steps, probe_bytes),
                                cand_kernel: (cand_ms / steps, cand_bytes)},
                               cand_kernel if cand_ms >= probe_ms else "k_query_probe8",
-                              f"{cfg.name.split('-')[0]}/{args.form}"),
+                              traffic_key or f"{cfg.name.split('-')[0]}/{args.form}"),
     }
     return out, gpu_ids
+
+
+def c3_precision(args, c3, n_check=300):
+    """c3 in the precision form (its workload's form): BF16x3 index
+    projection, FP16-row distance pass, FP64 rechecks; rechecked fractions,
+    ids and distances of a spread sample against the oracle's c3 index, and
+    the distances' error against FP64 (numpy)."""
+    import numpy as np
+    import torch
+
+    from culsh_amd import FORM_PRECISION, HashTables, SearchTables
+    cfg = c3.cfg
+    ht = HashTables()
+    assert ht.reset(cfg.N, cfg.D, cfg.L, cfg.M, cfg.W, c3.A, c3.b), ht.last_error
+    assert ht.set_form(FORM_PRECISION), ht.last_error
+    first_ms, index_ms = _index_twice(ht, c3.dX)
+    out, _ = bench_search_cfg(args, cfg, c3.X, c3.Y, c3.A, c3.b, ht, index_ms, FORM_PRECISION,
+                              max(3, args.steps // 2), "c3/precision")
+    out["index_first_ms"] = round(first_ms, 2)
+    # The counts above come from a host-API search of the host array, which
+    # reads the FP32 rows; rebuilt from that array, the index's resident rows
+    # are the host array and its search takes the FP16 distance pass, as the
+    # timed device searches did: its FP64 re-evaluations and algorithmic
+    # bytes (2 D per unique row, 4 D per re-evaluation) replace them.
+    assert ht.index(c3.X), ht.last_error
+    st = SearchTables()
+    assert st.reset(ht, cfg.K, cfg.T), st.last_error
+    assert st.set_form(FORM_PRECISION), st.last_error
+    assert st.search(c3.Y, c3.Y.shape[0], c3.X), st.last_error
+    cnu, cu = st.getQueryCandidates()
+    out["search_rechecked"] = st.getRechecked()
+    pb, cb = _search_bytes(cfg.D, cfg.L, cfg.T, cfg.K, cnu, cu, st.getRechecked(), 2 * cfg.D)
+    out["roofline"] = _roofline({"k_query_probe8": (out["probe_ms"], pb),
+                                 "k_candidates": (out["candidates_ms"], cb)},
+                                "k_candidates", "c3/precision")
+    out["index_rechecked_fraction"] = round(
+        out["index_rechecked"] / (cfg.N * cfg.L * cfg.M), 6)
+    out["search_rechecked_fraction"] = round(
+        out["search_rechecked"] / max(1, out["total_candidates"]["unique"]), 6)
+    sel = np.linspace(0, c3.Y.shape[0] - 1, n_check).astype(np.int64)
+    Ys = np.ascontiguousarray(c3.Y[sel])
+    dq = torch.from_numpy(Ys).cuda()
+    ids = torch.empty((n_check, cfg.K), dtype=torch.int32, device="cuda")
+    ds = torch.empty((n_check, cfg.K), dtype=torch.float32, device="cuda")
+    assert st.search_device(dq, ids, ds), st.last_error
+    assert st.wait(), st.last_error
+    gi, gd = ids.cpu().numpy(), ds.cpu().numpy()
+    o = c3.oracle().search(Ys, c3.X, cfg.K, cfg.T)
+    valid = gi >= 0
+    d64 = np.zeros(gi.shape, np.float64)
+    for q in range(n_check):
+        v = gi[q][valid[q]]
+        diff = c3.X[v].astype(np.float64) - Ys[q].astype(np.float64)
+        d64[q, :v.size] = (diff * diff).sum(axis=1)
+    err = np.abs(gd.astype(np.float64) - d64)[valid]
+    ulp = np.spacing(gd.astype(np.float32))[valid].astype(np.float64)
+    out["oracle_check"] = {
+        "queries": n_check, "ids_equal": bool(np.array_equal(gi, o["ids"])),
+        "dists_equal": bool(np.array_equal(gd, o["dists"])),
+        "max_abs_dist_err_vs_fp64": float(err.max()) if err.size else 0.0,
+        "max_dist_err_in_fp32_ulps": round(float((err / ulp).max()), 4) if err.size else 0.0}
+    out["workload"] = cfg.describe() + " (precision form)"
+    del st, ht
+    return out
 
 
 def bench_search(args):
@@ -668,6 +738,11 @@ def bench_all(args):
     if args.cpu_queries > 0:
         out["cpu_baseline"] = c4_cpu_baseline(c3, b, args.cpu_queries)
         log(f"cpu baseline {out['cpu_baseline']['value']} q/s")
+        # c3 in its own (precision) form, checked against the oracle's index.
+        out["c3_precision"] = c3_precision(args, c3)
+        oc = out["c3_precision"]["oracle_check"]
+        log(f"c3 precision {out['c3_precision']['value']:.0f} q/s, index "
+            f"{out['c3_precision']['index_ms']} ms, ids equal oracle {oc['ids_equal']}")
     # Free the c3 index before the serving indexes.
     del c3, b
     torch.cuda.empty_cache()
