@@ -207,11 +207,14 @@ def test_config3_slice():
 
 @pytest.mark.parametrize("st_form", [1, 2])
 @pytest.mark.parametrize("case", ["small_T8", "small_T40_K33", "ties", "c1", "c3_slice",
-                                  "large"])
+                                  "large", "coarse_half", "d_not_multiple_of_8"])
 def test_precision_form_matches_oracle(case, st_form):
     """Precision form: BF16x3 MFMA projection for the index; the search with
-    direct distances (1) or grouped norm-expansion MFMA distances (2); FP64
-    rechecks keep every result bit-exact."""
+    the distance pass on the FP16 copy of the resident rows (1) or grouped
+    norm-expansion MFMA distances (2); FP64 rechecks keep every result
+    bit-exact.  coarse_half: values near 1000, whose half spacing (0.5) is
+    large next to the distances (wide FP64 bands); d_not_multiple_of_8: no
+    FP16 copy, FP32 rows."""
     print("precision", case, "search form", st_form, flush=True)
     if case in ("small_T8", "small_T40_K33"):
         X, Y, A, b = _small(91)
@@ -231,6 +234,16 @@ def test_precision_form_matches_oracle(case, st_form):
         cfg = wl.CONFIGS["c3"]
         X, Y, A, b = wl.make_data(cfg, N=400_000, Q=1_000)
         L, M, W, K, T = cfg.L, cfg.M, cfg.W, cfg.K, cfg.T
+    elif case == "coarse_half":
+        X = (wl.mixture(8000, 24, 20, 0.2, 84) + 1000.0).astype(np.float32)
+        Y = wl.queries_near(X, 150, 0.05, 85)
+        A, b = wl.projection(5, 24, 8, 1.5, 86)
+        L, M, W, K, T = 5, 8, 1.5, 20, 16
+    elif case == "d_not_multiple_of_8":
+        X = wl.mixture(8000, 20, 20, 0.2, 87)
+        Y = wl.queries_near(X, 150, 0.05, 88)
+        A, b = wl.projection(5, 20, 8, 1.5, 89)
+        L, M, W, K, T = 5, 8, 1.5, 20, 16
     else:
         X = wl.mixture(30000, 16, 3, 0.3, 81)
         Y = wl.queries_near(X, 40, 0.05, 82)
