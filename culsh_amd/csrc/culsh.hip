@@ -1385,8 +1385,10 @@ int launch_probe_and_candidates(culsh_st* st, const float* dQ, const float* dX, 
 // global-memory kernel for queries that overflow it).  Synchronises to check
 // the per-query capacity flags, unless `wait` is false and the distances are
 // direct (plain and precision forms).
+// `buckets`: keep each probe's bucket index (the host search's query
+// buckets); device searches expose none and skip those writes.
 int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids, float* d_dists,
-               hipStream_t s, bool timed, bool wait = true) {
+               hipStream_t s, bool timed, bool wait = true, bool buckets = true) {
   TRY(st_check_async(st, wait));
   // A device search still running on another stream uses the same working
   // buffers: order this one after it.
@@ -1400,7 +1402,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   const int L = ht->L, D = ht->D, T = st->T, K = st->K;
   const long long QT = (long long)Q * T;
   if ((long long)L * QT >= (1ll << 31)) return fail(CULSH_E_LIMIT, "L*Q*T exceeds 2^31 - 1");
-  TRY(st->dqb.ensure((size_t)L * QT));
+  if (buckets) TRY(st->dqb.ensure((size_t)L * QT));
   TRY(st->dqr.ensure((size_t)L * QT));
   TRY(st->dcnu_qt.ensure((size_t)Q * L));
   TRY(st->dcnu_q.ensure(Q));
@@ -1411,7 +1413,7 @@ int run_search(culsh_st* st, const float* dQ, const float* dX, int Q, int* d_ids
   TRY(st->dmax.ensure(1));
   HIP_TRY(hipMemsetAsync(st->dnov.p, 0, 4, s));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, st->dcnt.p, 1);
-  SearchBufs b{st->dqb.p,  st->dqr.p,  st->dcnu_qt.p, st->dcnu_q.p,
+  SearchBufs b{buckets ? st->dqb.p : nullptr, st->dqr.p, st->dcnu_qt.p, st->dcnu_q.p,
                st->dcu_q.p, st->dov.p, st->dnov.p,   st->dcnt.p,
                true,       st->stamp ? st->dphase.p : nullptr};
   TRY(launch_probe_and_candidates(st, dQ, dX, Q, d_ids, d_dists, s, b, timed));
@@ -1495,7 +1497,10 @@ int search_pass(culsh_st* st, int Q, bool host, int* out) {
   culsh_ht* ht = st->ht;
   const long long L = ht->L, T = st->T, K = st->K, D = ht->D;
   long long q = std::min<long long>(Q, ((1ll << 31) - 1) / (L * T));
-  const long long per = 12 * L * T + 8 * L + 16 + 4 + 48 + (host ? 4 * L * T + 4 * D + 8 * K : 0);
+  // Per query: member ranges (8 L T), C_nu per table, counters; the host
+  // search also keeps bucket indices and stages queries, results and
+  // lexicographic query buckets.
+  const long long per = 8 * L * T + 8 * L + 16 + 4 + 48 + (host ? 8 * L * T + 4 * D + 8 * K : 0);
   size_t freeb = 0, totalb = 0;
   HIP_TRY(hipMemGetInfo(&freeb, &totalb));
   long long held =
@@ -1648,7 +1653,7 @@ int culsh_st_search_device(culsh_st* st, const float* d_queries, int Q, int* d_k
   for (int q0 = 0; q0 < Q; q0 += pass) {
     const int qn = std::min(pass, Q - q0);
     TRY(run_search(st, d_queries + (size_t)q0 * D, st->ht->dX, qn, d_knn_ids + (size_t)q0 * K,
-                   d_knn_dists + (size_t)q0 * K, (hipStream_t)stream, true, false));
+                   d_knn_dists + (size_t)q0 * K, (hipStream_t)stream, true, false, false));
   }
   return CULSH_OK;
 }
