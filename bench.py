@@ -90,10 +90,13 @@ def _roofline(kernels, dom, workload_key=None):
 
 
 def _search_bytes(D, L, T, K, cnu, cu, rechecked=0, row_bytes=None):
-    """Algorithmic bytes of a search: probe (query read + one key and offset
-    pair per probe) and candidates (bucket id lists, one read of every unique
-    row -- row_bytes, 4 D for the FP32 rows, 2 D for the batches' FP16 copy --
-    the FP32 rows re-read by the FP64 pass, results)."""
+    """Bytes of a search: probe (query read + one key and offset pair per
+    probe) and candidates (bucket id lists, one read of every unique row,
+    results).  With the defaults this is the algorithmic work per query of
+    the f32 layout, B_q = 4 D + 16 L T + 4 C_nu + 4 D C_u + 8 K, the same for
+    every form; row_bytes (2 D for an FP16 copy of the rows) and rechecked
+    (FP32 rows re-read by the FP64 pass, 4 D each) give the bytes a form
+    actually reads."""
     import numpy as np
     cnu = np.asarray(cnu, np.float64)
     cu = np.asarray(cu, np.float64)
@@ -225,7 +228,7 @@ def bench_search_cfg(args, cfg, X, Y, A, b, ht, index_ms, form, steps, traffic_k
     assert st.search(Y, Q, X), st.last_error
     assert np.array_equal(st.getKnnIds(), gpu_ids)
     cnu, cu = st.getQueryCandidates()
-    probe_bytes, cand_bytes = _search_bytes(D, L, T, K, cnu, cu, st.getRechecked())
+    probe_bytes, cand_bytes = _search_bytes(D, L, T, K, cnu, cu)
     from culsh_amd import FORM_PRECISION_GROUPED
     cand_kernel = "k_group_candidates" if form == FORM_PRECISION_GROUPED else "k_candidates"
     out = {
@@ -272,10 +275,14 @@ def c3_precision(args, c3, n_check=300):
     assert st.search(c3.Y, c3.Y.shape[0], c3.X), st.last_error
     cnu, cu = st.getQueryCandidates()
     out["search_rechecked"] = st.getRechecked()
-    pb, cb = _search_bytes(cfg.D, cfg.L, cfg.T, cfg.K, cnu, cu, st.getRechecked(), 2 * cfg.D)
+    pb, cb = _search_bytes(cfg.D, cfg.L, cfg.T, cfg.K, cnu, cu)
+    _, cb_read = _search_bytes(cfg.D, cfg.L, cfg.T, cfg.K, cnu, cu, st.getRechecked(),
+                               2 * cfg.D)
     out["roofline"] = _roofline({"k_query_probe8": (out["probe_ms"], pb),
                                  "k_candidates": (out["candidates_ms"], cb)},
                                 "k_candidates", "c3/precision")
+    out["roofline"]["read_model_bytes"] = cb_read
+    out["roofline"]["read_model_GBps"] = round(cb_read / (out["candidates_ms"] * 1e6), 2)
     out["index_rechecked_fraction"] = round(
         out["index_rechecked"] / (cfg.N * cfg.L * cfg.M), 6)
     out["search_rechecked_fraction"] = round(
@@ -451,21 +458,24 @@ def bench_batch_on(args, c3, steps):
     assert all(p is not None for p in prof), bt.last_error
     stage = {k: sum(p[k] for p in prof) / len(prof) for k in prof[0]}
 
-    # Algorithmic bytes of the batch from each request's candidate counts;
-    # the batch reads the FP16 copy of the rows (D % 8 == 0 and every value
-    # within the half range, as for this data) and re-reads the FP32 rows of
-    # the candidates its FP64 pass re-evaluates.
+    # Algorithmic bytes of the batch (the f32-layout work per query) from
+    # each request's candidate counts, and the bytes the batch actually
+    # reads: the FP16 copy of the rows (D % 8 == 0 and every value within
+    # the half range, as for this data) plus the FP32 rows of the candidates
+    # its FP64 pass re-evaluates.
     batch_rechecked = bt.getRechecked()
-    pb = cb = 0.0
+    pb = cb = cb_read = 0.0
     for Q, K, T, row, off in reqs:
         assert st.reset(ht, K, T)
         assert st.search(Y[row:row + Q], Q, c3.X), st.last_error
         cnu, cu = st.getQueryCandidates()
-        p_, c_ = _search_bytes(cfg.D, cfg.L, T, K, cnu, cu, 0,
-                               2 * cfg.D if cfg.D % 8 == 0 else 4 * cfg.D)
+        p_, c_ = _search_bytes(cfg.D, cfg.L, T, K, cnu, cu)
+        _, r_ = _search_bytes(cfg.D, cfg.L, T, K, cnu, cu, 0,
+                              2 * cfg.D if cfg.D % 8 == 0 else 4 * cfg.D)
         pb += p_
         cb += c_
-    cb += 4.0 * cfg.D * batch_rechecked
+        cb_read += r_
+    cb_read += 4.0 * cfg.D * batch_rechecked
     sched = bt.get_schedule()
     cand_name = "k_batch_cand" if not sched["fused"] else "k_batch"
     out = {
@@ -492,10 +502,14 @@ def bench_batch_on(args, c3, steps):
         "graph_ms": round(elapsed / steps * 1e3, 4),
         "_ids": h, "_reqs": reqs, "_Y": Y,
     }
+    out["roofline"]["read_model_bytes"] = cb_read
+    out["roofline"]["read_model_GBps"] = round(cb_read / (stage["candidates"] * 1e6), 2)
     out["roofline"]["note"] = ("kernel time: eager runs of the batch (same kernels, HIP events "
-                               "on the launch stream); bytes: candidate-stage algorithmic bytes "
-                               "of all requests (4 C_nu + 2 D C_u + 8 K per query: the FP16 "
-                               "copy of the rows) + 4 D per FP64 re-evaluation")
+                               "on the launch stream); algorithmic bytes: the candidate stage's "
+                               "f32-layout work of all requests (4 C_nu + 4 D C_u + 8 K per "
+                               "query, as for the plain path); read_model_bytes: what the batch "
+                               "reads (4 C_nu + 2 D C_u + 8 K with the FP16 copy of the rows, "
+                               "+ 4 D per FP64 re-evaluation); traffic: measured HBM bytes")
 
     out["autotune"]["second_mix"] = second_mix
     return out
