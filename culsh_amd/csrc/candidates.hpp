@@ -381,7 +381,18 @@ __device__ inline bool finish_topk(const float* __restrict__ X, const float* y, 
     s_ovf = 0;
   }
   double thr = INFINITY;
-  if (U > K) {
+  if (U > K && K == 1) {
+    // The smallest approximate distance: a block minimum of the float bits
+    // (non-negative, so their integer order is their numeric order).
+    uint32_t m = 0xFFFFFFFFu;
+    for (int u = tid; u < U; u += 256) m = min(m, __float_as_uint(fmaxf(dapx[u], 0.f)));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((tid & 63) == 0) s_misc[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) s_prefix = min(min(s_misc[0], s_misc[1]), min(s_misc[2], s_misc[3]));
+    __syncthreads();
+  } else if (U > K) {
     if (tid == 0) {
       s_prefix = 0;
       s_rem = (unsigned)K;
@@ -409,6 +420,8 @@ __device__ inline bool finish_topk(const float* __restrict__ X, const float* y, 
       pmask |= 255u << shiftb;
       __syncthreads();
     }
+  }
+  if (U > K) {
     const double dk = (double)__uint_as_float(s_prefix);
     if (e2 > 0.0) {
       // |sqrt(d~) - sqrt(d)| <= ||x - x~|| = e, so |d~ - d| <= 2 e sqrt(d) + e^2
