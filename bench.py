@@ -490,6 +490,10 @@ def bench_batch_on(args, c3, steps):
         "chosen_vs_plain": round(value / (rows / plain_s), 3),
         "results_equal_plain": bool(same),
         "fp64_rechecked": int(batch_rechecked),
+        "distance_rows": ("FP16 copy of the rows for the distance pass (widened exactly to "
+                          "f32), every candidate that can reach the top K re-evaluated in "
+                          "FP64 from the f32 rows: results bit-identical to the plain path "
+                          "(results_equal_plain) and the FP64 oracle"),
         "autotune": {"naive_default_ms": round(tune[0], 3), "tuned_ms": round(tune[1], 3),
                      "naive_default": {"probe_queries": 32, "probe_tables": 1,
                                        "queries_per_item": 32, "rows_in_flight": 4,
