@@ -153,9 +153,11 @@ int culsh_st_wait(culsh_st* st);
 /* Allocation-free variant of culsh_st_search_device.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory -- code
  * under HIP graph capture, tight serving loops, harnesses that forbid
- * hipMalloc.  Same kernels and results as culsh_st_search_device (plain form
- * only; no locality reordering of large query sets, which changes no
- * result), but every working buffer lives in one fixed 64 MiB device array
+ * hipMalloc.  Same kernels and results as culsh_st_search_device (not the
+ * grouped form; the precision form reads the FP16 row copy only when an
+ * earlier search built it; no locality reordering of large query sets,
+ * which changes no result), but every working buffer lives in one fixed
+ * 64 MiB device array
  * of the library (one per GPU, kept until the process ends), so the call
  * allocates nothing and never synchronises.  Limits: the working set of
  * the call (about 20 + 8 L + 8 L T bytes per query, plus 8 MiB) must fit the
