@@ -724,25 +724,28 @@ __global__ void k_table_bases(const uint32_t* __restrict__ flag, const uint32_t*
 // Probe index (see ProbeIndex), built from the key-sorted records.
 //
 // Key filter: the records of one filter word are consecutive (the word is
-// monotone in the key), so the first record of each run ORs the run's bits
-// and stores the word; the words start zeroed.
-__device__ inline uint32_t rec_filter_word(const BucketRec* __restrict__ recs, long long g, int tb,
-                                           const uint2* __restrict__ tfw) {
-  const uint64_t key = recs[g].key;
-  return filter_word(key, tb, tfw[(int)(key >> (64 - tb))]);
-}
-
+// monotone in the key), so a wave ORs the bits of each run of equal words
+// among its lanes (a segmented scan: lanes o apart with equal words hold
+// only that word's records between them) and the last lane of each run ORs
+// them into the word; the words start zeroed.
 __global__ __launch_bounds__(256) void k_filter(const BucketRec* __restrict__ recs, long long B,
                                                 int tb, const uint2* __restrict__ tfw,
                                                 uint64_t* __restrict__ filter) {
   const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (g >= B) return;
-  const uint32_t w = rec_filter_word(recs, g, tb, tfw);
-  if (g > 0 && rec_filter_word(recs, g - 1, tb, tfw) == w) return;
-  uint64_t bits = 0;
-  for (long long r = g; r < B && rec_filter_word(recs, r, tb, tfw) == w; ++r)
-    bits |= filter_bits(recs[r].key);
-  filter[w] = bits;
+  const bool in = g < B;
+  const uint64_t key = in ? recs[g].key : 0ull;
+  const uint32_t w = in ? filter_word(key, tb, tfw[(int)(key >> (64 - tb))]) : 0xffffffffu;
+  unsigned long long bits = in ? filter_bits(key) : 0ull;
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t wo = __shfl_up(w, o, 64);
+    const unsigned long long bo = __shfl_up(bits, o, 64);
+    if (lane >= o && wo == w) bits |= bo;
+  }
+  const uint32_t wn = __shfl_down(w, 1, 64);
+  if (in && (lane == 63 || wn != w))
+    atomicOr(reinterpret_cast<unsigned long long*>(filter) + w, bits);
 }
 
 // Key directory (see ProbeIndex) in one pass over the key-sorted records:
