@@ -29,7 +29,7 @@ Default (--config all), the metric on its own workloads, all on one GPU:
                     HBM and with the query upload + result download
   cpu_baseline      the oracle (FP64 OpenMP restatement of the reference
                     algorithm) on the host cores: search of the c4 requests
-                    (all 256 by default, about 5 s a run on 16 cores)
+                    (all 256 by default, about 7.5 s a run on 16 cores)
                     against its own c3 index (median of 3), index build
                     timed separately
 
